@@ -1,0 +1,69 @@
+// lbm_common.hpp — structures shared by the host runtime and the gfx950
+// kernels of the D2Q13 thermal FD-LBM.
+//
+// Device layout of one instance (a grid of nx*ny cells):
+//   * distributions: 26 planes (f0..f12, h0..h12), each ny rows of `pitch`
+//     cells, x fastest — structure of arrays, so a wavefront reading one
+//     direction of consecutive cells issues one coalesced row segment;
+//   * cell type map: uint8 [ny][pitch] (0 fluid, 1 permanent, 2 solid);
+//   * wall closure factor alpha: double [ny][pitch] (read at solid cells).
+// Storage is FP64 or FP32 (mixed precision) per instance group.
+#pragma once
+
+#include <cstdint>
+
+namespace uqlbm {
+
+constexpr int kQ = 13;
+constexpr int kPlanes = 26;
+
+// Boundary flag bits (reference cpuLBM.h:47-49).
+enum : int {
+  kPeriodicX = 1,
+  kPeriodicY = 2,
+  kMirrorN = 4,
+  kMirrorS = 8,
+  kMirrorE = 16,
+  kMirrorW = 32,
+};
+
+// Per-instance record read by every kernel. Plain data; lives in device
+// memory as an array indexed by instance id.
+struct InstDev {
+  int nx, ny, pitch, flags;
+  int tiles_x, tiles_y;
+  int flux_method;     // 0 NND, 1 WENO5
+  int parity0;         // which state buffer holds the current state at launch step 0
+  int step_base;       // steps taken before the current launch sequence
+  long long plane;     // elements per plane (pitch*ny)
+  void* state[2];      // ping-pong state buffers (26 planes each)
+  void* f2;            // stage-2 distributions (26 planes)
+  void* f3;            // stage-3 distributions (26 planes)
+  const uint8_t* ctype;
+  double* alpha;       // wall closure factor, [ny][pitch]
+  int* fail;           // per-instance: first failing step (INT_MAX when none)
+  // physics (reference setupVars, cpuLBM.cpp:360-450)
+  double dt, dx, dy;
+  double rdx, rdy;     // dt/dx, dt/dy
+  double c;            // lattice speed sqrt(R*Tc)
+  double sRTc;         // sqrt(R*Tc) as the equilibrium computes it
+  double Tc, R, b, K, bR, RTc;
+  double mu, Pr;
+};
+
+// One tile of one instance: the unit of work of a block.
+struct TileRef {
+  int inst;
+  int tx, ty;
+  int flags;           // kTileSolid: the tile's halo region holds a solid cell
+};
+
+enum : int { kTileSolid = 1 };
+
+// Wall cell that needs a closure factor (has a non-solid axis neighbour).
+struct WallRef {
+  int inst;
+  int x, y;
+};
+
+}  // namespace uqlbm

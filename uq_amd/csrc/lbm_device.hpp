@@ -1,0 +1,216 @@
+// lbm_device.hpp — gfx950 device arithmetic of the D2Q13 thermal FD-LBM:
+// lattice tables, equilibria, moments, WENO5 and NND face reconstructions.
+//
+// Compute type TC is double (FP64 path) or float (mixed path). Moments and
+// the WENO weight normalisation are always evaluated in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbm_common.hpp"
+
+namespace uqlbm {
+
+// Lattice velocities in units of c (reference setupVars, cpuLBM.cpp:376-388).
+__host__ __device__ constexpr int lat_x(int i) {
+  return i == 1 ? 1 : i == 3 ? -1 : i == 5 ? 1 : i == 6 ? -1 : i == 7 ? -1 : i == 8 ? 1
+       : i == 9 ? 2 : i == 11 ? -2 : 0;
+}
+__host__ __device__ constexpr int lat_y(int i) {
+  return i == 2 ? 1 : i == 4 ? -1 : i == 5 ? 1 : i == 6 ? 1 : i == 7 ? -1 : i == 8 ? -1
+       : i == 10 ? 2 : i == 12 ? -2 : 0;
+}
+__host__ __device__ constexpr int sgn_i(int a) { return a > 0 ? 1 : (a < 0 ? -1 : 0); }
+
+// Direction remaps for reflections about the x axis (north/south mirror) and
+// the y axis (east/west mirror) (reference mirIndex, cpuLBM.cpp:1147-1169).
+__host__ __device__ constexpr int mirror_ns(int i) {
+  return i == 2 ? 4 : i == 4 ? 2 : i == 5 ? 8 : i == 6 ? 7 : i == 7 ? 6 : i == 8 ? 5
+       : i == 10 ? 12 : i == 12 ? 10 : i;
+}
+__host__ __device__ constexpr int mirror_ew(int i) {
+  return i == 1 ? 3 : i == 3 ? 1 : i == 5 ? 6 : i == 6 ? 5 : i == 7 ? 8 : i == 8 ? 7
+       : i == 9 ? 11 : i == 11 ? 9 : i;
+}
+
+// Equilibrium argument map: direction i evaluates base polynomial
+// eq_base(i) at (a, b) where a = sa*(swap ? v : u), b = sb*(swap ? u : v)
+// (reference equilibrium2D, cpuLBM.cpp:795-823).
+__host__ __device__ constexpr int eq_base(int i) { return i == 0 ? 0 : i <= 4 ? 1 : i <= 8 ? 5 : 9; }
+__host__ __device__ constexpr bool eq_swap(int i) { return i == 2 || i == 4 || i == 10 || i == 12; }
+__host__ __device__ constexpr int eq_sa(int i) {
+  return (i == 3 || i == 4 || i == 6 || i == 7 || i == 11 || i == 12) ? -1 : 1;
+}
+__host__ __device__ constexpr int eq_sb(int i) { return (i == 7 || i == 8) ? -1 : 1; }
+
+// ---------------------------------------------------------------------------
+// Equilibrium polynomials: the closed forms of the reference
+// (inlineFunctions.cpp:10-61), evaluated with products instead of pow().
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T poly_f0(T rho, T u, T v, T t) {
+  const T u2 = u * u, v2 = v * v;
+  return (rho / T(4)) * (T(4) + T(10) * (t * t) + u2 * u2 - T(5) * v2 + v2 * v2 +
+                         T(10) * t * (T(-1) + u2 + v2) + u2 * (T(-5) + T(4) * v2));
+}
+template <typename T>
+__device__ __forceinline__ T poly_f1(T rho, T a, T b, T t) {
+  const T a2 = a * a, b2 = b * b;
+  return (rho / T(6)) * (T(-6) * (t * t) - a * (T(1) + a) * (T(-4) + a2 + T(3) * b2) -
+                         t * (T(-4) + T(6) * a + T(9) * a2 + T(3) * b2));
+}
+template <typename T>
+__device__ __forceinline__ T poly_f5(T rho, T a, T b, T t) {
+  return (rho / T(4)) * ((t + a + a * a) * (t + b + b * b));
+}
+template <typename T>
+__device__ __forceinline__ T poly_f9(T rho, T a, T t) {
+  return (rho / T(24)) * (T(3) * (t * t) + (T(-1) + a) * a * (T(1) + a) * (T(2) + a) +
+                          t * (T(-1) + T(6) * a * (T(1) + a)));
+}
+template <typename T>
+__device__ __forceinline__ T poly_h0(T K, T t, T u, T v) {
+  const T u2 = u * u, v2 = v * v, u4 = u2 * u2, v4 = v2 * v2, t2 = t * t, t3 = t2 * t;
+  return (T(10) * (T(16) + T(3) * K) * t3 +
+          T(3) * t *
+              (T(8) + T(4) * K - T(40) * u2 - T(5) * K * u2 + T(20) * u4 + K * u4 +
+               (T(-5) * (T(8) + K) + T(4) * (T(15) + K) * u2) * v2 + (T(20) + K) * v4) +
+          T(30) * t2 * (T(-4) + T(9) * u2 + T(9) * v2 + K * (T(-1) + u2 + v2)) +
+          T(3) * (u2 + v2) * (T(4) + u4 - T(5) * v2 + v4 + u2 * (T(-5) + T(4) * v2))) /
+         T(24);
+}
+template <typename T>
+__device__ __forceinline__ T poly_h1(T K, T t, T a, T b) {
+  const T a2 = a * a, a3 = a2 * a, b2 = b * b, t2 = t * t, t3 = t2 * t;
+  return (T(-2) * (T(16) + T(3) * K) * t3 - a * (T(1) + a) * (a2 + b2) * (T(-4) + a2 + T(3) * b2) -
+          t * (a * (T(-4) * (T(4) + K) - T(4) * (T(7) + K) * a + (T(14) + K) * a2 + (T(19) + K) * a3) +
+               (T(-4) + T(3) * a * (T(10) + K + (T(14) + K) * a)) * b2 + T(3) * (b2 * b2)) -
+          t2 * (T(-16) + T(6) * a * (T(6) + T(13) * a) + T(30) * b2 +
+                K * (T(-4) + T(6) * a + T(9) * a2 + T(3) * b2))) /
+         T(12);
+}
+template <typename T>
+__device__ __forceinline__ T poly_h5(T K, T t, T a, T b) {
+  const T a2 = a * a, b2 = b * b, t2 = t * t, t3 = t2 * t;
+  return ((T(16) + T(3) * K) * t3 + T(3) * a * (T(1) + a) * b * (T(1) + b) * (a2 + b2) +
+          T(3) * t2 * ((T(6) + K) * a + (T(9) + K) * a2 + b * (T(6) + K + (T(9) + K) * b)) +
+          T(3) * t *
+              (a2 * a + a2 * a2 + b2 * b * (T(1) + b) + a * b * (T(6) + K + (T(9) + K) * b) +
+               a2 * b * (T(9) + K + (T(12) + K) * b))) /
+         T(24);
+}
+template <typename T>
+__device__ __forceinline__ T poly_h9(T K, T t, T a, T b) {
+  const T a2 = a * a, b2 = b * b, t2 = t * t, t3 = t2 * t;
+  return ((T(16) + T(3) * K) * t3 +
+          t * a * (T(-8) + K * (T(-1) + a) * (T(1) + a) * (T(2) + a) + a * (T(-7) + T(2) * a * (T(11) + T(8) * a))) +
+          t * (T(-1) + T(6) * a * (T(1) + a)) * b2 + (T(-1) + a) * a * (T(1) + a) * (T(2) + a) * (a2 + b2) +
+          t2 * (T(-4) + T(36) * a + T(51) * a2 + K * (T(-1) + T(6) * a * (T(1) + a)) + T(3) * b2)) /
+         T(48);
+}
+
+// Normalised equilibrium inputs of one cell.
+template <typename T>
+struct EqIn {
+  T rho;    // density
+  T u, v;   // velocity / sqrt(R*Tc)
+  T t;      // temperature / Tc
+  T K;      // internal degrees of freedom
+  T scale;  // rho*R*Tc, the h scale
+};
+
+// f and h equilibrium of direction I (compile-time).
+template <int I, typename T>
+__device__ __forceinline__ void eq_dir(const EqIn<T>& q, T& fe, T& he) {
+  constexpr int base = eq_base(I);
+  const T a = T(eq_sa(I)) * (eq_swap(I) ? q.v : q.u);
+  const T b = T(eq_sb(I)) * (eq_swap(I) ? q.u : q.v);
+  if constexpr (base == 0) {
+    fe = poly_f0(q.rho, q.u, q.v, q.t);
+    he = q.scale * poly_h0(q.K, q.t, q.u, q.v);
+  } else if constexpr (base == 1) {
+    fe = poly_f1(q.rho, a, b, q.t);
+    he = q.scale * poly_h1(q.K, q.t, a, b);
+  } else if constexpr (base == 5) {
+    fe = poly_f5(q.rho, a, b, q.t);
+    he = q.scale * poly_h5(q.K, q.t, a, b);
+  } else {
+    fe = poly_f9(q.rho, a, q.t);
+    he = q.scale * poly_h9(q.K, q.t, a, b);
+  }
+}
+
+// Moments of one cell (reference macroProp, cpuLBM.cpp:916-970), FP64.
+struct Macro {
+  double rho, ux, uy, T, p, tf, th, thf;
+};
+
+__device__ __forceinline__ void finish_macro(const InstDev& in, double rho, double rux, double ruy,
+                                             double sh, Macro& m) {
+  m.rho = rho;
+  m.ux = rux / rho;
+  m.uy = ruy / rho;
+  const double ax = fabs(m.ux), ay = fabs(m.uy);
+  const double usq = ax * ax + ay * ay;
+  m.T = 2.0 * (sh / rho - usq / 2.0) / in.bR;
+  m.p = rho * in.R * m.T;
+  m.tf = in.mu / m.p;
+  m.th = m.tf / in.Pr;
+  m.thf = (m.th * m.tf) / (m.tf - m.th);
+}
+
+template <typename T>
+__device__ __forceinline__ EqIn<T> eq_inputs(const InstDev& in, const Macro& m) {
+  EqIn<T> q;
+  q.rho = T(m.rho);
+  q.u = T(m.ux / in.sRTc);
+  q.v = T(m.uy / in.sRTc);
+  q.t = T(m.T / in.Tc);
+  q.K = T(in.K);
+  q.scale = T(m.rho * in.RTc);
+  return q;
+}
+
+// ---------------------------------------------------------------------------
+// Face reconstructions (reference WENO5 cpuLBM.cpp:1358-1403, NND
+// :1328-1356, minmod :1130-1145). `v` is |e| along the axis (non-zero).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T minmod(T a, T b) {
+  if ((fabs(a) < fabs(b)) && (a * b > T(0))) return a;
+  if ((fabs(b) < fabs(a)) && (a * b > T(0))) return b;
+  if (a == b) return a;
+  if (a * b <= T(0)) return T(0);
+  return a + b;  // NaN input only
+}
+
+template <typename T>
+__device__ __forceinline__ T nnd(T v, T s0, T s1, T s2) {
+  const T fi = v * s1, fp = v * s2, fm = v * s0;
+  return fi + T(0.5) * minmod(fp - fi, fi - fm);
+}
+
+// Jiang-Shu WENO5 with d = (0.1, 0.6, 0.3), eps = 1e-6 on |e|*S. The
+// normalised weights w_k = a_k / sum(a) with a_k = d_k/(eps+b_k)^2 are formed
+// as A_k / sum(A) with A_k = d_k * prod_{j!=k} (eps+b_j)^2 — the same ratio,
+// one division instead of six — in FP64 for both precisions.
+template <typename T>
+__device__ __forceinline__ T weno5(T v, T s0, T s1, T s2, T s3, T s4) {
+  const T q0 = s0 * v, q1 = s1 * v, q2 = s2 * v, q3 = s3 * v, q4 = s4 * v;
+  const T c0 = q0 - T(2) * q1 + q2, d0 = q0 - T(4) * q1 + T(3) * q2;
+  const T c1 = q1 - T(2) * q2 + q3, d1 = q1 - q3;
+  const T c2 = q2 - T(2) * q3 + q4, d2 = T(3) * q2 - T(4) * q3 + q4;
+  const T b0 = T(13.0 / 12.0) * (c0 * c0) + T(0.25) * (d0 * d0);
+  const T b1 = T(13.0 / 12.0) * (c1 * c1) + T(0.25) * (d1 * d1);
+  const T b2 = T(13.0 / 12.0) * (c2 * c2) + T(0.25) * (d2 * d2);
+  const T r0 = T(2.0 / 6.0) * q0 - T(7.0 / 6.0) * q1 + T(11.0 / 6.0) * q2;
+  const T r1 = -T(1.0 / 6.0) * q1 + T(5.0 / 6.0) * q2 + T(2.0 / 6.0) * q3;
+  const T r2 = T(2.0 / 6.0) * q2 + T(5.0 / 6.0) * q3 - T(1.0 / 6.0) * q4;
+  const double e0 = 1e-6 + double(b0), e1 = 1e-6 + double(b1), e2 = 1e-6 + double(b2);
+  const double e00 = e0 * e0, e11 = e1 * e1, e22 = e2 * e2;
+  const double A0 = 0.1 * (e11 * e22), A1 = 0.6 * (e00 * e22), A2 = 0.3 * (e00 * e11);
+  const double num = A0 * double(r0) + A1 * double(r1) + A2 * double(r2);
+  return T(num / (A0 + A1 + A2));
+}
+
+}  // namespace uqlbm

@@ -1,0 +1,84 @@
+// lbm_launch.hpp — host-side launch interface between the runtime and the
+// kernels of lbm_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "lbm_common.hpp"
+
+namespace uqlbm {
+
+// Device pointers of one batched step: the instance table and the work lists.
+struct StepLaunch {
+  const InstDev* insts = nullptr;
+  const TileRef* tiles = nullptr;
+  int ntiles = 0;
+  const WallRef* walls = nullptr;
+  int nwalls = 0;
+};
+
+// Optional per-kernel timing with HIP events recorded on the launch stream
+// around each launch; elapsed times are collected after a synchronise.
+class KernelTimer {
+ public:
+  static constexpr int kSlots = 8;
+  void set_names(const std::vector<std::string>& names) { names_ = names; }
+  void begin(int k, hipStream_t s) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(a, s);
+    open_[k] = std::make_pair(a, b);
+  }
+  void end(int k, hipStream_t s) {
+    auto ev = open_[k];
+    (void)hipEventRecord(ev.second, s);
+    pending_.push_back({k, ev});
+  }
+  // Collect elapsed times of all recorded pairs (call after synchronising).
+  void collect() {
+    for (auto& p : pending_) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+        total_ms_[p.first] += ms;
+        count_[p.first] += 1;
+      }
+      (void)hipEventDestroy(p.second.first);
+      (void)hipEventDestroy(p.second.second);
+    }
+    pending_.clear();
+  }
+  void reset() {
+    collect();
+    for (int k = 0; k < kSlots; k++) { total_ms_[k] = 0; count_[k] = 0; }
+  }
+  double avg_ms(int k) const { return count_[k] ? total_ms_[k] / count_[k] : 0.0; }
+  long long count(int k) const { return count_[k]; }
+  const std::vector<std::string>& names() const { return names_; }
+
+ private:
+  std::vector<std::string> names_;
+  std::pair<hipEvent_t, hipEvent_t> open_[kSlots] = {};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_;
+  double total_ms_[kSlots] = {};
+  long long count_[kSlots] = {};
+};
+
+int tile_w();
+int tile_h();
+int tile_halo();
+
+template <typename TS, typename TC>
+void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm);
+
+template <typename TS>
+void launch_init(const InstDev* insts, int inst, const int* ids, const double* props, long long cells,
+                 hipStream_t s);
+
+template <typename TS>
+void launch_macro(const InstDev* insts, int inst, int par, double* out, long long cells, hipStream_t s);
+
+}  // namespace uqlbm

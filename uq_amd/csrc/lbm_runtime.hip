@@ -1,0 +1,824 @@
+// lbm_runtime.hip — host runtime and C ABI (include/uqlbm.h) of the MI355X
+// D2Q13 thermal FD-LBM: deck refinement, device state, work lists, batched
+// stepping, geometry validation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/uqlbm.h"
+#include "lbm_common.hpp"
+#include "lbm_device.hpp"
+#include "lbm_launch.hpp"
+#include "lbm_runtime.hpp"
+
+namespace uqlbm {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_error;
+
+int fail_with(int status, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return status;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail_with(LBM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+static thread_local KernelTimer* g_timer = nullptr;
+static bool g_timing_enabled = false;
+static thread_local lbm_timing g_last_timing = {};
+
+// ---------------------------------------------------------------------------
+// deck refinement (reference refineInput / setupVars, cpuLBM.cpp:283-450)
+// ---------------------------------------------------------------------------
+static void refine_deck(Solver& s) {
+  lbm_params& p = s.p;
+  if (p.dx == 0) p.dx = p.Lx / ((double)s.nx - 1);
+  else p.Lx = p.dx * (s.nx - 1);
+  if (p.dy == 0 && p.Ly != 0) p.dy = p.Ly / ((double)s.ny - 1);
+  else if (p.dy == 0 && p.Ly == 0) { p.dy = p.dx; p.Ly = p.dy * (s.ny - 1); }
+  else p.Ly = p.dy * (s.ny - 1);
+  if (p.rho_ref == 0) p.rho_ref = p.p_ref / (p.R * p.T_ref);
+  else if (p.p_ref == 0) p.p_ref = p.rho_ref * p.R * p.T_ref;
+  else if (p.T_ref == 0) p.T_ref = p.p_ref / (p.rho_ref * p.R);
+  for (auto& r : s.props) {
+    if (r.rho == 0) r.rho = r.p / (p.R * r.T);
+    else if (r.p == 0) r.p = r.rho * p.R * r.T;
+    else if (r.T == 0) r.T = r.p / (r.rho * p.R);
+  }
+  s.b = 2 / (p.gamma - 1);
+  s.K = s.b - 2;
+  s.Tc = p.Tc_Tref * p.T_ref;
+  const double tau_ref = p.mu / p.p_ref;
+  const double u_ref = std::sqrt(p.R * p.T_ref);
+  const double t_ref = std::max(p.Lx, p.Ly) / u_ref;
+  s.c = std::sqrt(p.R * s.Tc);
+  const double e9 = 2 * s.c;
+  s.dt = p.dtau == 0 ? (p.cfl * std::min(p.dx, p.dy)) / e9 : p.dtau * tau_ref;
+  double total = 0;
+  bool need = false;
+  s.nsteps = 0;
+  if (p.tt_tref > 0) { total = p.tt_tref * t_ref; need = true; }
+  else if (p.tt_time > 0) { total = p.tt_time; need = true; }
+  else if (p.steps > 0) s.nsteps = p.steps;
+  if (need) s.nsteps = (int)std::ceil(total / s.dt);
+}
+
+// ---------------------------------------------------------------------------
+// Geometry validation: the tile kernels resolve a stencil point from its
+// logical position (cell + offset mapped once through the boundary rules,
+// direction remapped at mirrors) and take an upwind face flux across a
+// clamped edge from the cell itself. The reference instead maps the
+// physical neighbour of a physical point (extrapolation points, upwind
+// owners). The two agree except next to thin solid features or solids at a
+// mirror edge; this check proves agreement cell by cell for one geometry.
+// ---------------------------------------------------------------------------
+struct Pt {
+  int x, y, d;
+  bool operator==(const Pt& o) const { return x == o.x && y == o.y && d == o.d; }
+};
+
+static int ref_map(const Solver& s, int i, int axis, int& mir, bool& clamped) {
+  const int n = axis == 0 ? s.nx : s.ny;
+  const bool per = axis == 0 ? s.p.periodic_x > 0 : s.p.periodic_y > 0;
+  const bool mlo = axis == 0 ? s.p.mirror_west > 0 : s.p.mirror_south > 0;
+  const bool mhi = axis == 0 ? s.p.mirror_east > 0 : s.p.mirror_north > 0;
+  mir = 0;
+  clamped = false;
+  if (i < 0) {
+    if (per) i = n + i;
+    else if (mlo) { i = -i - 1; mir = 1; }
+    else { i = 0; clamped = true; }
+  } else if (i > n - 1) {
+    if (per) i = i - n;
+    else if (mhi) { i = 2 * n - i - 1; mir = 1; }
+    else { i = n - 1; clamped = true; }
+  }
+  return std::min(std::max(i, 0), n - 1);
+}
+
+static int type_at(const Solver& s, int x, int y) { return s.ctype_host[(size_t)y * s.nx + x]; }
+
+struct LineReads {
+  int L = 0;
+  Pt pts[5];
+  int sld[5] = {0, 0, 0, 0, 0};
+  bool any = false;
+  int jj = 0, diff = 0;
+  Pt e1{}, e2{};
+  bool wall_owner = false;
+  Pt wall_src{};
+};
+
+// Reference: out-face stencil of physical cell (cx, cy), direction i, axis.
+static void ref_line(const Solver& s, int cx, int cy, int i, int axis, LineReads& r) {
+  const int e = axis == 0 ? lat_x(i) : lat_y(i), sg = sgn_i(e);
+  r = LineReads();
+  if (type_at(s, cx, cy) == 2) {
+    r.wall_owner = true;
+    r.wall_src = {cx, cy, i};
+    return;
+  }
+  r.L = s.p.flux_method == 1 ? 5 : 3;
+  const int low = (r.L - 1) / 2;
+  const int start = (axis == 0 ? cx : cy) - sg * low;
+  for (int j = 0; j < r.L; j++) {
+    int mir;
+    bool cl;
+    const int p = ref_map(s, start + sg * j, axis, mir, cl);
+    const int d = mir ? (axis == 0 ? mirror_ew(i) : mirror_ns(i)) : i;
+    r.pts[j] = axis == 0 ? Pt{p, cy, d} : Pt{cx, p, d};
+    r.sld[j] = (axis == 0 ? type_at(s, p, cy) : type_at(s, cx, p)) == 2;
+    r.any |= r.sld[j] != 0;
+  }
+  if (!r.any) return;
+  int pos = 0;
+  for (int j = 0; j < r.L - 1; j++) {
+    const int d = r.sld[j] - r.sld[j + 1];
+    int mir;
+    bool cl;
+    if (d > 0) { r.diff = d; r.jj = j; pos = ref_map(s, start + sg * j, axis, mir, cl); break; }
+    if (d < 0) { r.diff = d; r.jj = j + 1; pos = ref_map(s, start + sg * (j + 1), axis, mir, cl); break; }
+  }
+  int mir;
+  bool cl;
+  const int m1 = ref_map(s, pos + sg * r.diff, axis, mir, cl);
+  const int m2 = ref_map(s, pos + sg * 2 * r.diff, axis, mir, cl);
+  r.e1 = axis == 0 ? Pt{m1, cy, i} : Pt{cx, m1, i};
+  r.e2 = axis == 0 ? Pt{m2, cy, i} : Pt{cx, m2, i};
+}
+
+// Kernel: out-face stencil of the owner at logical position (lx, ly) (one
+// coordinate may lie outside the grid), direction i, axis.
+static Pt ker_point(const Solver& s, int lx, int ly, int i) {
+  int mx, my;
+  bool cx, cy;
+  const int X = ref_map(s, lx, 0, mx, cx);
+  const int Y = ref_map(s, ly, 1, my, cy);
+  int d = i;
+  if (mx) d = mirror_ew(d);
+  if (my) d = mirror_ns(d);
+  return {X, Y, d};
+}
+
+static void ker_line(const Solver& s, int lx, int ly, int i, int axis, LineReads& r) {
+  const int e = axis == 0 ? lat_x(i) : lat_y(i), sg = sgn_i(e);
+  r = LineReads();
+  const Pt own = ker_point(s, lx, ly, i);
+  if (type_at(s, own.x, own.y) == 2) {
+    r.wall_owner = true;
+    r.wall_src = own;
+    return;
+  }
+  r.L = s.p.flux_method == 1 ? 5 : 3;
+  const int low = (r.L - 1) / 2;
+  for (int k = 0; k < r.L; k++) {
+    const int px = axis == 0 ? lx + (k - low) * sg : lx;
+    const int py = axis == 0 ? ly : ly + (k - low) * sg;
+    r.pts[k] = ker_point(s, px, py, i);
+    r.sld[k] = type_at(s, r.pts[k].x, r.pts[k].y) == 2;
+    r.any |= r.sld[k] != 0;
+  }
+  if (!r.any) return;
+  for (int j = 0; j < r.L - 1; j++) {
+    const int d = r.sld[j] - r.sld[j + 1];
+    if (d != 0) { r.diff = d; r.jj = d > 0 ? j : j + 1; break; }
+  }
+  const int k1 = std::min(std::max(r.jj + r.diff, 0), r.L - 1);
+  const int k2 = std::min(std::max(r.jj + 2 * r.diff, 0), r.L - 1);
+  r.e1 = r.pts[k1];
+  r.e2 = r.pts[k2];
+}
+
+static bool same(const LineReads& a, const LineReads& b) {
+  if (a.wall_owner != b.wall_owner) return false;
+  if (a.wall_owner) return a.wall_src == b.wall_src;
+  if (a.L != b.L || a.any != b.any) return false;
+  for (int k = 0; k < a.L; k++)
+    if (!(a.pts[k] == b.pts[k]) || a.sld[k] != b.sld[k]) return false;
+  if (!a.any) return true;
+  return a.jj == b.jj && a.diff == b.diff && a.e1 == b.e1 && a.e2 == b.e2;
+}
+
+static int validate_geometry(const Solver& s) {
+  const int R = 5;  // reach of owner (1) + stencil (2) + extrapolation (2)
+  for (int axis = 0; axis < 2; axis++) {
+    const int n = axis == 0 ? s.nx : s.ny, m = axis == 0 ? s.ny : s.nx;
+    std::vector<char> near(n);
+    for (int line = 0; line < m; line++) {
+      std::fill(near.begin(), near.end(), 0);
+      for (int k = 0; k < n; k++) {
+        const bool edge = k < R || k >= n - R;
+        const int t = axis == 0 ? type_at(s, k, line) : type_at(s, line, k);
+        if (edge) near[k] = 1;
+        if (t == 2)
+          for (int q = std::max(0, k - R); q <= std::min(n - 1, k + R); q++) near[q] = 1;
+      }
+      for (int k = 0; k < n; k++) {
+        if (!near[k]) continue;
+        const int cx = axis == 0 ? k : line, cy = axis == 0 ? line : k;
+        if (type_at(s, cx, cy) == 2) continue;
+        for (int i = 1; i < kQ; i++) {
+          const int e = axis == 0 ? lat_x(i) : lat_y(i);
+          if (e == 0) continue;
+          const int sg = sgn_i(e);
+          LineReads a, b;
+          // own out-face
+          ref_line(s, cx, cy, i, axis, a);
+          ker_line(s, cx, cy, i, axis, b);
+          if (!same(a, b))
+            return fail_with(LBM_ERR_GEOMETRY,
+                             "geometry not supported by the tile kernels: stencil of cell (%d,%d) "
+                             "direction %d axis %d differs from the reference addressing",
+                             cx, cy, i, axis);
+          // upwind owner (reference cFLUX: index(x,-s) with mirror remap)
+          int mir;
+          bool cl;
+          const int up = ref_map(s, (axis == 0 ? cx : cy) - sg, axis, mir, cl);
+          const int iu = mir ? (axis == 0 ? mirror_ew(i) : mirror_ns(i)) : i;
+          if (cl) {
+            if (up != (axis == 0 ? cx : cy))
+              return fail_with(LBM_ERR_GEOMETRY, "clamped upwind owner of (%d,%d) is not the cell itself",
+                               cx, cy);
+            continue;  // the kernel uses the cell's own out-face
+          }
+          ref_line(s, axis == 0 ? up : cx, axis == 0 ? cy : up, iu, axis, a);
+          ker_line(s, axis == 0 ? cx - sg : cx, axis == 0 ? cy : cy - sg, i, axis, b);
+          if (!same(a, b))
+            return fail_with(LBM_ERR_GEOMETRY,
+                             "geometry not supported by the tile kernels: upwind face of cell (%d,%d) "
+                             "direction %d axis %d differs from the reference addressing",
+                             cx, cy, i, axis);
+        }
+      }
+    }
+  }
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// solver lifetime
+// ---------------------------------------------------------------------------
+static size_t elem_size(int precision) { return precision == LBM_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+Solver::~Solver() {
+  for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
+                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev})
+    if (p) (void)hipFree(p);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static void fill_inst(const Solver& s, InstDev& d) {
+  d = InstDev{};
+  d.nx = s.nx;
+  d.ny = s.ny;
+  d.pitch = s.pitch;
+  d.flags = (s.p.periodic_x > 0 ? kPeriodicX : 0) | (s.p.periodic_y > 0 ? kPeriodicY : 0) |
+            (s.p.mirror_north > 0 ? kMirrorN : 0) | (s.p.mirror_south > 0 ? kMirrorS : 0) |
+            (s.p.mirror_east > 0 ? kMirrorE : 0) | (s.p.mirror_west > 0 ? kMirrorW : 0);
+  d.tiles_x = (s.nx + tile_w() - 1) / tile_w();
+  d.tiles_y = (s.ny + tile_h() - 1) / tile_h();
+  d.flux_method = s.p.flux_method;
+  d.parity0 = s.cur;
+  d.step_base = s.step;
+  d.plane = (long long)s.pitch * s.ny;
+  d.state[0] = s.state[0];
+  d.state[1] = s.state[1];
+  d.f2 = s.f2;
+  d.f3 = s.f3;
+  d.ctype = s.ctype;
+  d.alpha = s.alpha;
+  d.fail = s.fail;
+  d.dt = s.dt;
+  d.dx = s.p.dx;
+  d.dy = s.p.dy;
+  d.rdx = s.dt / s.p.dx;
+  d.rdy = s.dt / s.p.dy;
+  d.c = s.c;
+  d.sRTc = std::sqrt(s.p.R * s.Tc);
+  d.Tc = s.Tc;
+  d.R = s.p.R;
+  d.b = s.b;
+  d.K = s.K;
+  d.bR = s.b * s.p.R;
+  d.RTc = s.p.R * s.Tc;
+  d.mu = s.p.mu;
+  d.Pr = s.p.Pr;
+}
+
+static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls, int inst) {
+  const int TW = tile_w(), TH = tile_h(), H = tile_halo();
+  const int tx_n = (s.nx + TW - 1) / TW, ty_n = (s.ny + TH - 1) / TH;
+  // prefix count of solid cells for the per-tile solid flag
+  std::vector<int> pre((size_t)(s.nx + 1) * (s.ny + 1), 0);
+  auto P = [&](int x, int y) -> int& { return pre[(size_t)y * (s.nx + 1) + x]; };
+  for (int y = 0; y < s.ny; y++)
+    for (int x = 0; x < s.nx; x++)
+      P(x + 1, y + 1) = P(x, y + 1) + P(x + 1, y) - P(x, y) + (type_at(s, x, y) == 2);
+  auto count = [&](int x0, int y0, int x1, int y1) {
+    x0 = std::max(x0, 0); y0 = std::max(y0, 0);
+    x1 = std::min(x1, s.nx); y1 = std::min(y1, s.ny);
+    if (x0 >= x1 || y0 >= y1) return 0;
+    return P(x1, y1) - P(x0, y1) - P(x1, y0) + P(x0, y0);
+  };
+  const bool any_edge_solid = count(0, 0, s.nx, H + 1) || count(0, s.ny - H - 1, s.nx, s.ny) ||
+                              count(0, 0, H + 1, s.ny) || count(s.nx - H - 1, 0, s.nx, s.ny);
+  for (int ty = 0; ty < ty_n; ty++)
+    for (int tx = 0; tx < tx_n; tx++) {
+      const int x0 = tx * TW, y0 = ty * TH;
+      bool solid = count(x0 - H, y0 - H, x0 + TW + H, y0 + TH + H) > 0;
+      // halo positions wrapped or mirrored across an edge may reach solids
+      // on the far side: be conservative on edge tiles
+      const bool edge = x0 - H < 0 || y0 - H < 0 || x0 + TW + H > s.nx || y0 + TH + H > s.ny;
+      if (edge && any_edge_solid) solid = true;
+      tiles.push_back(TileRef{inst, tx, ty, solid ? kTileSolid : 0});
+    }
+  for (int y = 0; y < s.ny; y++)
+    for (int x = 0; x < s.nx; x++) {
+      if (type_at(s, x, y) != 2) continue;
+      static const int dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};
+      bool need = false;
+      for (int j = 0; j < 4 && !need; j++) {
+        int m;
+        bool cl;
+        const int xn = ref_map(s, x + dxs[j], 0, m, cl);
+        const int yn = ref_map(s, y + dys[j], 1, m, cl);
+        need = type_at(s, xn, yn) < 2;
+      }
+      if (need) walls.push_back(WallRef{inst, x, y});
+    }
+}
+
+}  // namespace uqlbm
+
+using namespace uqlbm;
+
+extern "C" const char* lbm_last_error(void) { return g_error.c_str(); }
+
+extern "C" int32_t lbm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  int usable = 0;
+  for (int d = 0; d < n; d++) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, d) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) usable++;
+  }
+  return usable;
+}
+
+extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, const int32_t* bnd, const lbm_props* props,
+               int32_t precision, lbm_solver** out) {
+  if (!params || !bnd || !props || !out) return fail_with(LBM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (nx < 4 || ny < 4) return fail_with(LBM_ERR_ARG, "grid %dx%d too small (need at least 4x4)", nx, ny);
+  if (precision != LBM_PREC_FP64 && precision != LBM_PREC_MIXED)
+    return fail_with(LBM_ERR_ARG, "unknown precision %d", precision);
+  if (params->rk_method != 1)
+    return fail_with(LBM_ERR_ARG, "RKMETHOD %d not supported on the device path (only 1)", params->rk_method);
+  if (params->flux_method != 0 && params->flux_method != 1)
+    return fail_with(LBM_ERR_ARG, "FMETHOD %d unknown", params->flux_method);
+  if (props->num_props < 1 || !props->density || !props->pressure || !props->temp || !props->velocity_x ||
+      !props->velocity_y || !props->cell_type)
+    return fail_with(LBM_ERR_ARG, "incomplete property table");
+  std::unique_ptr<Solver> s(new Solver());
+  s->p = *params;
+  s->nx = nx;
+  s->ny = ny;
+  s->precision = precision;
+  s->props.resize(props->num_props);
+  for (int r = 0; r < props->num_props; r++) {
+    s->props[r] = {props->density[r], props->pressure[r], props->temp[r], props->velocity_x[r],
+                   props->velocity_y[r], props->cell_type[r]};
+    if (props->cell_type[r] < 0 || props->cell_type[r] > 2)
+      return fail_with(LBM_ERR_ARG, "cellType %d of property %d out of range", props->cell_type[r], r);
+  }
+  refine_deck(*s);
+  if (!(s->dt > 0) || !std::isfinite(s->dt)) return fail_with(LBM_ERR_ARG, "derived time step is not positive");
+
+  // host maps in [y][x]
+  s->ids_host.assign((size_t)nx * ny, 0);
+  s->ctype_host.assign((size_t)nx * ny, 0);
+  for (int x = 0; x < nx; x++)
+    for (int y = 0; y < ny; y++) {
+      const int id = bnd[(size_t)x * ny + y];
+      if (id < 0 || id >= props->num_props)
+        return fail_with(LBM_ERR_ARG, "bndArray(%d,%d) = %d is not a property id", x, y, id);
+      s->ids_host[(size_t)y * nx + x] = id;
+      s->ctype_host[(size_t)y * nx + x] = (uint8_t)s->props[id].type;
+    }
+  int st = validate_geometry(*s);
+  if (st != LBM_OK) return st;
+  if (lbm_device_count() < 1) return fail_with(LBM_ERR_NOGPU, "no gfx950 device visible");
+
+  const int align = 32;
+  s->pitch = (nx + align - 1) / align * align;
+  const size_t plane = (size_t)s->pitch * ny;
+  const size_t es = elem_size(precision);
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  for (void** b : {&s->state[0], &s->state[1], &s->f2, &s->f3}) HIP_TRY(hipMalloc(b, plane * kPlanes * es));
+  HIP_TRY(hipMalloc((void**)&s->ctype, plane));
+  HIP_TRY(hipMalloc((void**)&s->alpha, plane * sizeof(double)));
+  HIP_TRY(hipMalloc((void**)&s->fail, sizeof(int)));
+  HIP_TRY(hipMalloc((void**)&s->ids, plane * sizeof(int)));
+  HIP_TRY(hipMalloc((void**)&s->props_dev, s->props.size() * 4 * sizeof(double)));
+  HIP_TRY(hipMalloc((void**)&s->inst_dev, sizeof(InstDev)));
+  HIP_TRY(hipMalloc((void**)&s->macro_dev, plane * 5 * sizeof(double)));
+  s->device_bytes = plane * kPlanes * es * 4 + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
+
+  std::vector<uint8_t> ct(plane, 2);
+  std::vector<int> ids(plane, 0);
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      ct[(size_t)y * s->pitch + x] = s->ctype_host[(size_t)y * nx + x];
+      ids[(size_t)y * s->pitch + x] = s->ids_host[(size_t)y * nx + x];
+    }
+  HIP_TRY(hipMemcpy(s->ctype, ct.data(), plane, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->ids, ids.data(), plane * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<double> pr(s->props.size() * 4);
+  for (size_t r = 0; r < s->props.size(); r++) {
+    pr[4 * r + 0] = s->props[r].rho;
+    pr[4 * r + 1] = s->props[r].ux;
+    pr[4 * r + 2] = s->props[r].uy;
+    pr[4 * r + 3] = s->props[r].T;
+  }
+  HIP_TRY(hipMemcpy(s->props_dev, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(s->alpha, 0, plane * sizeof(double)));
+  for (void* b : {s->state[0], s->state[1], s->f2, s->f3}) HIP_TRY(hipMemset(b, 0, plane * kPlanes * es));
+
+  std::vector<TileRef> tiles;
+  std::vector<WallRef> walls;
+  build_lists(*s, tiles, walls, 0);
+  s->ntiles = (int)tiles.size();
+  s->nwalls = (int)walls.size();
+  HIP_TRY(hipMalloc((void**)&s->tiles_dev, tiles.size() * sizeof(TileRef)));
+  HIP_TRY(hipMemcpy(s->tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (!walls.empty()) {
+    HIP_TRY(hipMalloc((void**)&s->walls_dev, walls.size() * sizeof(WallRef)));
+    HIP_TRY(hipMemcpy(s->walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
+  }
+  s->tiles_host = std::move(tiles);
+  s->walls_host = std::move(walls);
+  st = upload_inst(*s);
+  if (st != LBM_OK) return st;
+  *out = reinterpret_cast<lbm_solver*>(s.release());
+  return LBM_OK;
+}
+
+extern "C" int lbm_init(lbm_solver* h) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  s.cur = 0;
+  s.step = 0;
+  int st = upload_inst(s);
+  if (st != LBM_OK) return st;
+  const int none = INT_MAX;
+  HIP_TRY(hipMemcpyAsync(s.fail, &none, sizeof(int), hipMemcpyHostToDevice, s.stream));
+  const long long cells = (long long)s.pitch * s.ny;
+  if (s.precision == LBM_PREC_FP64) launch_init<double>(s.inst_dev, 0, s.ids, s.props_dev, cells, s.stream);
+  else launch_init<float>(s.inst_dev, 0, s.ids, s.props_dev, cells, s.stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  s.inited = true;
+  return LBM_OK;
+}
+
+int uqlbm::upload_inst(Solver& s) {
+  InstDev d;
+  fill_inst(s, d);
+  HIP_TRY(hipMemcpyAsync(s.inst_dev, &d, sizeof(InstDev), hipMemcpyHostToDevice, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return LBM_OK;
+}
+
+static int macro_into(Solver& s, std::vector<double>& host) {
+  const long long cells = (long long)s.pitch * s.ny;
+  if (s.precision == LBM_PREC_FP64) launch_macro<double>(s.inst_dev, 0, 0, s.macro_dev, cells, s.stream);
+  else launch_macro<float>(s.inst_dev, 0, 0, s.macro_dev, cells, s.stream);
+  HIP_TRY(hipGetLastError());
+  host.resize((size_t)cells * 5);
+  HIP_TRY(hipMemcpyAsync(host.data(), s.macro_dev, host.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return LBM_OK;
+}
+
+static void launch_steps(const StepLaunch& L, int precision, hipStream_t stream, int nsteps,
+                         KernelTimer* tm) {
+  for (int k = 0; k < nsteps; k++) {
+    const int par = k & 1;
+    if (precision == LBM_PREC_FP64) launch_rk3_step<double, double>(L, stream, par, k + 1, tm);
+    else launch_rk3_step<float, float>(L, stream, par, k + 1, tm);
+  }
+}
+
+static KernelTimer* active_timer() {
+  if (!g_timing_enabled) return nullptr;
+  if (!g_timer) {
+    g_timer = new KernelTimer();
+    g_timer->set_names({"stage12", "alpha", "stage3", "final"});
+  }
+  return g_timer;
+}
+
+static void record_timing(double total_ms, KernelTimer* tm) {
+  g_last_timing = lbm_timing{};
+  g_last_timing.total_ms = total_ms;
+  if (tm) {
+    tm->collect();
+    const auto& names = tm->names();
+    g_last_timing.n_kernels = (int)names.size();
+    for (size_t k = 0; k < names.size() && k < 8; k++) {
+      std::snprintf(g_last_timing.names[k], 32, "%s", names[k].c_str());
+      g_last_timing.avg_ms[k] = tm->avg_ms((int)k);
+      g_last_timing.launches[k] = tm->count((int)k);
+    }
+    tm->reset();
+  }
+}
+
+// Advance every member of a group by n steps with batched launches: upload
+// the instance table (current buffer, step base), launch, synchronise, then
+// account each member's steps, stopping a member at its failing step
+// (reference main loop stop on runOne failure, main.cpp:21,56,73).
+int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, const StepLaunch& L, int precision,
+              hipStream_t stream, int n) {
+  std::vector<InstDev> table(members.size());
+  for (size_t k = 0; k < members.size(); k++) fill_inst(*members[k], table[k]);
+  HIP_TRY(hipMemcpyAsync(table_dev, table.data(), table.size() * sizeof(InstDev), hipMemcpyHostToDevice, stream));
+  KernelTimer* tm = active_timer();
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, stream));
+  launch_steps(L, precision, stream, n, tm);
+  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  record_timing(ms, tm);
+  int status = LBM_OK;
+  for (Solver* s : members) {
+    int fail_step = INT_MAX;
+    HIP_TRY(hipMemcpyAsync(&fail_step, s->fail, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (fail_step != INT_MAX) {
+      const int taken = std::max(0, fail_step - s->step);
+      s->cur ^= taken & 1;
+      s->step += taken;
+      s->failed = true;
+      status = LBM_FAILED;
+    } else {
+      s->cur ^= n & 1;
+      s->step += n;
+    }
+  }
+  return status;
+}
+
+extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, double* residual_out) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_step before lbm_init");
+  if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
+  if (s.failed) return LBM_FAILED;
+  if (nsteps == 0) return LBM_OK;
+  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls};
+  std::vector<Solver*> one{&s};
+  int st = LBM_OK;
+  if (want_residual && nsteps > 1) {
+    st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps - 1);
+    if (st != LBM_OK) {
+      (void)upload_inst(s);
+      return st;
+    }
+  }
+  std::vector<double> before;
+  if (want_residual) {
+    st = upload_inst(s);
+    if (st == LBM_OK) st = macro_into(s, before);
+    if (st != LBM_OK) return st;
+  }
+  st = run_group(one, s.inst_dev, L, s.precision, s.stream, want_residual ? 1 : nsteps);
+  int st2 = upload_inst(s);
+  if (st < 0) return st;
+  if (st2 != LBM_OK) return st2;
+  if (want_residual && residual_out) {
+    // residual: RMS change of T over the last step (cpuLBM.cpp:2322-2331)
+    std::vector<double> after;
+    st2 = macro_into(s, after);
+    if (st2 != LBM_OK) return st2;
+    const size_t n = (size_t)s.pitch * s.ny;
+    double sum = 0;
+    for (int y = 0; y < s.ny; y++)
+      for (int x = 0; x < s.nx; x++) {
+        const size_t o = 4 * n + (size_t)y * s.pitch + x;
+        const double d = after[o] - before[o];
+        sum += d * d;
+      }
+    *residual_out = std::sqrt(sum / ((double)s.nx * s.ny));
+  }
+  return st;
+}
+
+extern "C" int lbm_get_macro(lbm_solver* h, double* rho, double* ux, double* uy, double* p, double* T) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_get_macro before lbm_init");
+  std::vector<double> m;
+  int st = macro_into(s, m);
+  if (st != LBM_OK) return st;
+  const size_t n = (size_t)s.pitch * s.ny;
+  double* outs[5] = {rho, ux, uy, p, T};
+  for (int k = 0; k < 5; k++) {
+    if (!outs[k]) continue;
+    for (int x = 0; x < s.nx; x++)
+      for (int y = 0; y < s.ny; y++) outs[k][(size_t)x * s.ny + y] = m[k * n + (size_t)y * s.pitch + x];
+  }
+  return LBM_OK;
+}
+
+template <typename TS>
+static int get_dist_t(Solver& s, double* f, double* hh) {
+  const size_t plane = (size_t)s.pitch * s.ny;
+  std::vector<TS> buf(plane * kPlanes);
+  HIP_TRY(hipMemcpyAsync(buf.data(), s.state[s.cur], buf.size() * sizeof(TS), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  for (int x = 0; x < s.nx; x++)
+    for (int y = 0; y < s.ny; y++)
+      for (int i = 0; i < kQ; i++) {
+        const size_t o = (size_t)y * s.pitch + x, k = ((size_t)x * s.ny + y) * kQ + i;
+        if (f) f[k] = double(buf[plane * i + o]);
+        if (hh) hh[k] = double(buf[plane * (kQ + i) + o]);
+      }
+  return LBM_OK;
+}
+
+template <typename TS>
+static int set_dist_t(Solver& s, const double* f, const double* hh) {
+  const size_t plane = (size_t)s.pitch * s.ny;
+  std::vector<TS> buf(plane * kPlanes);
+  HIP_TRY(hipMemcpyAsync(buf.data(), s.state[s.cur], buf.size() * sizeof(TS), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  for (int x = 0; x < s.nx; x++)
+    for (int y = 0; y < s.ny; y++)
+      for (int i = 0; i < kQ; i++) {
+        const size_t o = (size_t)y * s.pitch + x, k = ((size_t)x * s.ny + y) * kQ + i;
+        if (f) buf[plane * i + o] = TS(f[k]);
+        if (hh) buf[plane * (kQ + i) + o] = TS(hh[k]);
+      }
+  HIP_TRY(hipMemcpyAsync(s.state[s.cur], buf.data(), buf.size() * sizeof(TS), hipMemcpyHostToDevice, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return LBM_OK;
+}
+
+extern "C" int lbm_get_dist(lbm_solver* h, double* f, double* hh) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_get_dist before lbm_init");
+  return s.precision == LBM_PREC_FP64 ? get_dist_t<double>(s, f, hh) : get_dist_t<float>(s, f, hh);
+}
+
+extern "C" int lbm_set_dist(lbm_solver* h, const double* f, const double* hh) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_set_dist before lbm_init");
+  return s.precision == LBM_PREC_FP64 ? set_dist_t<double>(s, f, hh) : set_dist_t<float>(s, f, hh);
+}
+
+extern "C" int lbm_get_info(const lbm_solver* h, lbm_info* out) {
+  if (!h || !out) return fail_with(LBM_ERR_ARG, "null argument");
+  const Solver& s = *reinterpret_cast<const Solver*>(h);
+  *out = lbm_info{};
+  out->nx = s.nx;
+  out->ny = s.ny;
+  out->nsteps = s.nsteps;
+  out->step = s.step;
+  out->dt = s.dt;
+  out->c = s.c;
+  out->Tc = s.Tc;
+  out->b = s.b;
+  out->K = s.K;
+  out->dx = s.p.dx;
+  out->dy = s.p.dy;
+  out->tstep = s.step * s.dt;
+  out->precision = s.precision;
+  out->halo = tile_halo();
+  out->device_bytes = s.device_bytes;
+  return LBM_OK;
+}
+
+extern "C" void lbm_destroy(lbm_solver* h) { delete reinterpret_cast<Solver*>(h); }
+
+extern "C" int lbm_set_kernel_timing(int32_t enable) {
+  g_timing_enabled = enable != 0;
+  return LBM_OK;
+}
+
+extern "C" int lbm_get_timing(lbm_timing* out) {
+  if (!out) return fail_with(LBM_ERR_ARG, "null argument");
+  *out = g_last_timing;
+  return LBM_OK;
+}
+
+extern "C" double lbm_clock_s(void) {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+extern "C" int lbm_batch_create(lbm_solver* const* members, int32_t n, lbm_batch** out) {
+  if (!members || n < 1 || !out) return fail_with(LBM_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  std::unique_ptr<Batch> b(new Batch());
+  for (int k = 0; k < n; k++) {
+    Solver* s = reinterpret_cast<Solver*>(members[k]);
+    if (!s || !s->inited) return fail_with(LBM_ERR_STATE, "batch member %d not initialised", k);
+    if (k && (s->precision != b->precision || s->p.rk_method != b->rk_method))
+      return fail_with(LBM_ERR_ARG, "batch members differ in precision or RK method");
+    b->precision = s->precision;
+    b->rk_method = s->p.rk_method;
+    b->members.push_back(s);
+  }
+  return batch_finish(*b, out);
+}
+
+int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
+  std::vector<TileRef> tiles;
+  std::vector<WallRef> walls;
+  for (size_t k = 0; k < b.members.size(); k++) {
+    for (TileRef t : b.members[k]->tiles_host) {
+      t.inst = (int)k;
+      tiles.push_back(t);
+    }
+    for (WallRef w : b.members[k]->walls_host) {
+      w.inst = (int)k;
+      walls.push_back(w);
+    }
+  }
+  b.ntiles = (int)tiles.size();
+  b.nwalls = (int)walls.size();
+  HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
+  HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
+  HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (!walls.empty()) {
+    HIP_TRY(hipMalloc((void**)&b.walls_dev, walls.size() * sizeof(WallRef)));
+    HIP_TRY(hipMemcpy(b.walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
+  }
+  *out = reinterpret_cast<lbm_batch*>(new Batch(std::move(b)));
+  b.inst_dev = nullptr;
+  b.tiles_dev = nullptr;
+  b.walls_dev = nullptr;
+  b.stream = nullptr;
+  return LBM_OK;
+}
+
+uqlbm::Batch::~Batch() {
+  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev})
+    if (p) (void)hipFree(p);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int uqlbm::batch_step(Batch& b, int nsteps) {
+  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls};
+  int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps);
+  for (Solver* s : b.members) {
+    int st2 = upload_inst(*s);
+    if (st2 != LBM_OK) return st2;
+  }
+  return st;
+}
+
+extern "C" int lbm_batch_step(lbm_batch* hb, int32_t nsteps) {
+  if (!hb) return fail_with(LBM_ERR_ARG, "null batch");
+  if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
+  if (nsteps == 0) return LBM_OK;
+  return batch_step(*reinterpret_cast<Batch*>(hb), nsteps);
+}
+
+extern "C" void lbm_batch_destroy(lbm_batch* b) { delete reinterpret_cast<Batch*>(b); }
+

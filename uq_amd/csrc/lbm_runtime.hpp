@@ -1,0 +1,97 @@
+// lbm_runtime.hpp — host-side objects behind the opaque handles of
+// include/uqlbm.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/uqlbm.h"
+#include "lbm_common.hpp"
+#include "lbm_launch.hpp"
+
+namespace uqlbm {
+
+struct PropRow {
+  double rho, p, T, ux, uy;
+  int type;
+};
+
+// One solver instance: the deck, its derived constants and its device state.
+struct Solver {
+  lbm_params p{};
+  int nx = 0, ny = 0, pitch = 0;
+  int precision = LBM_PREC_FP64;
+  std::vector<PropRow> props;
+  double b = 0, K = 0, Tc = 0, c = 0, dt = 0;
+  int nsteps = 0;
+
+  std::vector<int> ids_host;        // [y][nx]
+  std::vector<uint8_t> ctype_host;  // [y][nx]
+  std::vector<TileRef> tiles_host;
+  std::vector<WallRef> walls_host;
+
+  hipStream_t stream = nullptr;
+  void* state[2] = {nullptr, nullptr};
+  void* f2 = nullptr;
+  void* f3 = nullptr;
+  uint8_t* ctype = nullptr;
+  double* alpha = nullptr;
+  int* fail = nullptr;
+  int* ids = nullptr;
+  double* props_dev = nullptr;
+  InstDev* inst_dev = nullptr;
+  TileRef* tiles_dev = nullptr;
+  WallRef* walls_dev = nullptr;
+  double* macro_dev = nullptr;
+  int ntiles = 0, nwalls = 0;
+  size_t device_bytes = 0;
+
+  int cur = 0;    // state buffer holding the current state
+  int step = 0;   // steps taken
+  bool inited = false;
+  bool failed = false;
+
+  ~Solver();
+};
+
+// A set of solvers advanced together by batched launches.
+struct Batch {
+  std::vector<Solver*> members;
+  int precision = LBM_PREC_FP64;
+  int rk_method = 1;
+  hipStream_t stream = nullptr;
+  InstDev* inst_dev = nullptr;
+  TileRef* tiles_dev = nullptr;
+  WallRef* walls_dev = nullptr;
+  int ntiles = 0, nwalls = 0;
+  Batch() = default;
+  Batch(Batch&& o) noexcept { *this = std::move(o); }
+  Batch& operator=(Batch&& o) noexcept {
+    members = std::move(o.members);
+    precision = o.precision;
+    rk_method = o.rk_method;
+    stream = o.stream;
+    inst_dev = o.inst_dev;
+    tiles_dev = o.tiles_dev;
+    walls_dev = o.walls_dev;
+    ntiles = o.ntiles;
+    nwalls = o.nwalls;
+    o.stream = nullptr;
+    o.inst_dev = nullptr;
+    o.tiles_dev = nullptr;
+    o.walls_dev = nullptr;
+    return *this;
+  }
+  ~Batch();
+};
+
+int fail_with(int status, const char* fmt, ...);
+int upload_inst(Solver& s);
+int batch_finish(Batch& b, lbm_batch** out);
+int batch_step(Batch& b, int nsteps);
+int run_group(const std::vector<Solver*>& members, InstDev* table_dev, const StepLaunch& L, int precision,
+              hipStream_t stream, int n);
+
+}  // namespace uqlbm
