@@ -1,0 +1,124 @@
+"""The C-ABI library loads, exports every entry point include/uqlbm.h
+declares, agrees with the ctypes struct mirrors, and validates decks on the
+host (CPU only: no compute call needs a GPU here)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from uq_amd import capi, cases, lbm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "uqlbm.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lbm.lib()
+    names = _declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header(tmp_path):
+    src = tmp_path / "sizes.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "uqlbm.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(lbm_params), sizeof(lbm_props),"
+        " sizeof(lbm_info), sizeof(lbm_timing), sizeof(lbm_request_result), offsetof(lbm_params, Tc_Tref),"
+        " offsetof(lbm_info, device_bytes)); return 0;}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(v) for v in out]
+    exp = [C.sizeof(capi.LbmParams), C.sizeof(capi.LbmProps), C.sizeof(capi.LbmInfo),
+           C.sizeof(capi.LbmTiming), C.sizeof(capi.LbmRequestResult),
+           capi.LbmParams.Tc_Tref.offset, capi.LbmInfo.device_bytes.offset]
+    assert got == exp
+
+
+def _create_status(case, precision=0):
+    d = capi.PackedDeck(case.params, case.bnd, case.props)
+    h = C.c_void_p()
+    st = lbm.lib().lbm_create(C.byref(d.params), case.nx, case.ny, d.bnd_ptr(), C.byref(d.props),
+                              precision, C.byref(h))
+    if st == 0:
+        lbm.lib().lbm_destroy(h)
+    return st, lbm.lib().lbm_last_error().decode()
+
+
+@pytest.mark.parametrize("case", [
+    cases.couette(64), cases.couette(37, nx=45), cases.shear_wave(40), cases.riemann(64),
+    cases.flat_plate(80, 40, plate_start=20), cases.flat_plate(512, 256),
+], ids=lambda c: c.name)
+def test_create_accepts_workload_geometries(case):
+    st, msg = _create_status(case)
+    assert st in (capi.LBM_OK, capi.LBM_ERR_NOGPU), msg
+
+
+def test_create_rejects_thin_solid_feature():
+    """A one-cell-thick solid strip inside the fluid makes the reference's
+    extrapolation reach outside the 3-cell halo; the library refuses it
+    instead of computing something different."""
+    c = cases.couette(32)
+    b = c.bnd.copy()
+    b[10, 5:20] = 1
+    c.bnd = b
+    st, msg = _create_status(c)
+    assert st == capi.LBM_ERR_GEOMETRY, msg
+    assert "geometry" in msg
+
+
+@pytest.mark.parametrize("mutate,expected", [
+    (lambda c: c.params.__setitem__("rk_method", 7), capi.LBM_ERR_ARG),
+    (lambda c: c.params.__setitem__("flux_method", 3), capi.LBM_ERR_ARG),
+    (lambda c: c.bnd.__setitem__((0, 0), 99), capi.LBM_ERR_ARG),
+])
+def test_create_rejects_bad_decks(mutate, expected):
+    c = cases.couette(16)
+    c.bnd = c.bnd.copy()
+    mutate(c)
+    st, msg = _create_status(c)
+    assert st == expected, msg
+
+
+def test_create_rejects_tiny_grid_and_bad_precision():
+    st, _ = _create_status(cases.couette(8, nx=3))
+    assert st == capi.LBM_ERR_ARG
+    st, _ = _create_status(cases.couette(16), precision=5)
+    assert st == capi.LBM_ERR_ARG
+
+
+def test_null_handles_are_rejected():
+    L = lbm.lib()
+    assert L.lbm_init(None) == capi.LBM_ERR_ARG
+    assert L.lbm_step(None, 1, 0, None) == capi.LBM_ERR_ARG
+    assert L.lbm_batch_step(None, 1) == capi.LBM_ERR_ARG
+
+
+def test_case_builders_are_consistent():
+    ens = cases.uq_ensemble(32, seed=5, steps=100)
+    assert len(ens) == 32
+    assert {c.nx for c in ens} <= {64, 128, 256, 512}
+    again = cases.uq_ensemble(32, seed=5, steps=100)
+    assert [c.meta["tau"] for c in ens] == [c.meta["tau"] for c in again]
+    for c in ens:
+        assert 0.05 <= c.meta["u_wall"] <= 0.2 and 0.1 <= c.meta["mach"] <= 0.5
+        T = c.meta["T"]
+        assert T == pytest.approx(c.meta["u_wall"] ** 2 / (1.4 * c.meta["mach"] ** 2))
+    reqs = cases.request_stream(rate=50, duration=4.0, seed=1)
+    assert 100 < len(reqs) < 320
+    assert all(64 <= r.case.nx <= 1024 and 500 <= r.case.nsteps <= 5000 for r in reqs)
+    assert all(reqs[k].arrival_s < reqs[k + 1].arrival_s for k in range(len(reqs) - 1))
+    r = cases.riemann(16)
+    assert np.array_equal(r.bnd, np.rot90(np.rot90(r.bnd)) * 0 + r.bnd)
+    assert r.bnd[12, 12] == 0 and r.bnd[2, 12] == 1 and r.bnd[2, 2] == 2 and r.bnd[12, 2] == 3

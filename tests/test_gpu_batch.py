@@ -1,0 +1,127 @@
+"""Batched stepping and the continuous-batching server on the GPU.
+
+Per-instance arithmetic does not depend on the batch: every member of a
+batch, and every request served in a mix, must equal the same solve run
+alone bit for bit.
+"""
+import numpy as np
+import pytest
+
+from uq_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_ensemble():
+    rng = np.random.default_rng(11)
+    out = []
+    for k, (kind, n) in enumerate([("couette", 40), ("shear", 48), ("couette", 72), ("shear", 33),
+                                   ("couette", 64), ("shear", 20)]):
+        s = cases.uq_sample(rng)
+        out.append(cases.uq_instance(kind, n, s, steps=30))
+    out.append(cases.flat_plate(nx=70, ny=34, plate_start=18, steps=30))
+    out.append(cases.riemann(50, steps=30))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_alone_bitwise(gpu_lib, precision):
+    lbm = gpu_lib
+    ens = _small_ensemble()
+    print(f"[batch] {len(ens)} members precision={precision}", flush=True)
+    alone = []
+    for c in ens:
+        s = lbm.Solver(c, precision)
+        assert s.step(30) == 0
+        alone.append(s.get_dist())
+        s.close()
+    solvers = [lbm.Solver(c, precision) for c in ens]
+    b = lbm.Batch(solvers)
+    assert b.step(12) == 0
+    assert b.step(18) == 0
+    for s, (fa, ha) in zip(solvers, alone):
+        f, h = s.get_dist()
+        assert s.info()["step"] == 30
+        assert np.array_equal(f, fa) and np.array_equal(h, ha)
+
+
+def test_batch_member_failure_is_isolated(gpu_lib):
+    lbm = gpu_lib
+    good = cases.couette(40, steps=10)
+    bad = cases.couette(40, steps=10)
+    bad.props = {k: v.copy() for k, v in bad.props.items()}
+    bad.props["density"][:] = 1e11       # equilibrium mass check trips
+    bad.props["pressure"][:] = 0.0
+    s_good, s_bad = lbm.Solver(good), lbm.Solver(bad)
+    ref = lbm.Solver(good)
+    ref.step(10)
+    b = lbm.Batch([s_good, s_bad])
+    st = b.step(10)
+    assert st == 1                       # LBM_FAILED
+    assert s_bad.info()["step"] == 1     # stopped at the failing step
+    assert s_good.info()["step"] == 10
+    fg, hg = s_good.get_dist()
+    fr, hr = ref.get_dist()
+    assert np.array_equal(fg, fr) and np.array_equal(hg, hr)
+    # a failed solver refuses further steps
+    assert s_bad.step(1) == 1
+
+
+def test_server_results_equal_alone(gpu_lib):
+    lbm = gpu_lib
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(7):
+        n = int(rng.integers(24, 90))
+        steps = int(rng.integers(5, 40))
+        kind = "couette" if k % 2 == 0 else "shear"
+        reqs.append(cases.uq_instance(kind, n, cases.uq_sample(rng), steps=steps))
+    sv = lbm.Server(precision="mixed", hbm_budget_bytes=1 << 34, max_slots=3, keep_results=True)
+    for k, c in enumerate(reqs[:4]):
+        sv.submit(c, tag=k)
+    results = []
+    active, queued = sv.tick(7)
+    results += sv.poll()
+    for k, c in enumerate(reqs[4:], start=4):   # arrivals while others are on the GPU
+        sv.submit(c, tag=k)
+    for _ in range(200):
+        active, queued = sv.tick(9)
+        results += sv.poll()
+        assert active <= 3
+        if not active and not queued:
+            break
+    assert sorted(r["tag"] for r in results) == list(range(len(reqs)))
+    for r in results:
+        c = reqs[r["tag"]]
+        assert r["status"] == 0 and r["steps_done"] == c.nsteps
+        assert r["finish_s"] >= r["admit_s"] >= r["submit_s"]
+        s = lbm.Solver(c, "mixed")
+        s.step(c.nsteps)
+        m = s.get_macro()
+        mm = sv.result_macro(r["tag"])
+        for k in ("rho", "ux", "uy", "T", "p"):
+            assert np.array_equal(m[k], mm[k]), (r["tag"], k)
+        s.close()
+    assert sv.stats()["hbm_in_use"] == 0
+
+
+def test_server_admits_by_memory_budget(gpu_lib):
+    lbm = gpu_lib
+    c = cases.couette(64, steps=6)
+    from uq_amd.capi import LBM_PREC_MIXED  # noqa: F401
+    one = lbm.Solver(c, "mixed")
+    fp = one.info()["device_bytes"]
+    one.close()
+    sv = lbm.Server(precision="mixed", hbm_budget_bytes=int(2.5 * fp), max_slots=64)
+    for k in range(6):
+        sv.submit(c, tag=k)
+    seen = 0
+    done = []
+    for _ in range(50):
+        active, queued = sv.tick(3)
+        seen = max(seen, active)
+        done += sv.poll()
+        if not active and not queued:
+            break
+    assert seen == 2
+    assert len(done) == 6 and all(r["status"] == 0 for r in done)

@@ -501,11 +501,11 @@ extern "C" int lbm_init(lbm_solver* h) {
   return LBM_OK;
 }
 
+// Refresh the solver's own instance record (current buffer, step base) on
+// its stream; later work on that stream sees it in order.
 int uqlbm::upload_inst(Solver& s) {
-  InstDev d;
-  fill_inst(s, d);
-  HIP_TRY(hipMemcpyAsync(s.inst_dev, &d, sizeof(InstDev), hipMemcpyHostToDevice, s.stream));
-  HIP_TRY(hipStreamSynchronize(s.stream));
+  fill_inst(s, s.inst_host);
+  HIP_TRY(hipMemcpyAsync(s.inst_dev, &s.inst_host, sizeof(InstDev), hipMemcpyHostToDevice, s.stream));
   return LBM_OK;
 }
 

@@ -42,6 +42,7 @@ struct Solver {
   int* ids = nullptr;
   double* props_dev = nullptr;
   InstDev* inst_dev = nullptr;
+  InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
   double* macro_dev = nullptr;
@@ -63,6 +64,7 @@ struct Batch {
   int rk_method = 1;
   hipStream_t stream = nullptr;
   InstDev* inst_dev = nullptr;
+  InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
   int ntiles = 0, nwalls = 0;

@@ -1,0 +1,331 @@
+// lbm_server.hip — continuous-batching request loop over the batched
+// stepper: host-side queue, admission by device footprint and slots, device-
+// resident state, batched rounds, retirement at each request's own step
+// count. The reference steps a single cpuLBM from its main loop
+// (main.cpp:21-24); this is the served form of that loop for many solves.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/uqlbm.h"
+#include "lbm_launch.hpp"
+#include "lbm_runtime.hpp"
+
+namespace uqlbm {
+
+struct PendingReq {
+  lbm_params params;
+  int nx, ny;
+  std::vector<int32_t> bnd;
+  std::vector<double> dens, pres, temp, vx, vy;
+  std::vector<int32_t> ctype;
+  int nsteps;
+  int64_t tag;
+  double submit_s;
+  size_t footprint;
+};
+
+struct ActiveReq {
+  Solver* solver;
+  int nsteps;
+  int64_t tag;
+  double submit_s, admit_s;
+  size_t footprint;
+};
+
+struct KeptMacro {
+  int nx, ny;
+  std::vector<double> rho, ux, uy, p, T;
+};
+
+struct Server {
+  int precision = LBM_PREC_MIXED;
+  int rk_method = 1;
+  size_t budget = 0;
+  int max_slots = 0;
+  bool keep = false;
+  std::deque<PendingReq> queue;
+  std::vector<ActiveReq> active;
+  std::vector<lbm_request_result> done;
+  std::map<int64_t, KeptMacro> kept;
+  size_t in_use = 0;
+  long long lups = 0;
+  hipStream_t stream = nullptr;
+  InstDev* table_dev = nullptr;
+  TileRef* tiles_dev = nullptr;
+  WallRef* walls_dev = nullptr;
+  size_t tiles_cap = 0, walls_cap = 0;
+  int ntiles = 0, nwalls = 0;
+  bool lists_dirty = true;
+  ~Server();
+};
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device bytes a solver of this size will hold (matches lbm_create).
+static size_t footprint_of(int nx, int ny, int precision) {
+  const size_t pitch = (size_t)((nx + 31) / 32 * 32);
+  const size_t plane = pitch * ny;
+  const size_t es = precision == LBM_PREC_FP64 ? 8 : 4;
+  return plane * kPlanes * es * 4 + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
+}
+
+Server::~Server() {
+  for (auto& a : active) delete a.solver;
+  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev})
+    if (p) (void)hipFree(p);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+#define SV_TRY(expr)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return fail_with(LBM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+static int rebuild_lists(Server& sv) {
+  std::vector<TileRef> tiles;
+  std::vector<WallRef> walls;
+  for (size_t k = 0; k < sv.active.size(); k++) {
+    for (TileRef t : sv.active[k].solver->tiles_host) {
+      t.inst = (int)k;
+      tiles.push_back(t);
+    }
+    for (WallRef w : sv.active[k].solver->walls_host) {
+      w.inst = (int)k;
+      walls.push_back(w);
+    }
+  }
+  if (tiles.size() > sv.tiles_cap) {
+    if (sv.tiles_dev) SV_TRY(hipFree(sv.tiles_dev));
+    sv.tiles_cap = std::max(tiles.size(), sv.tiles_cap * 2);
+    SV_TRY(hipMalloc((void**)&sv.tiles_dev, sv.tiles_cap * sizeof(TileRef)));
+  }
+  if (walls.size() > sv.walls_cap) {
+    if (sv.walls_dev) SV_TRY(hipFree(sv.walls_dev));
+    sv.walls_cap = std::max(walls.size(), sv.walls_cap * 2);
+    SV_TRY(hipMalloc((void**)&sv.walls_dev, sv.walls_cap * sizeof(WallRef)));
+  }
+  if (!tiles.empty())
+    SV_TRY(hipMemcpyAsync(sv.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
+                          sv.stream));
+  if (!walls.empty())
+    SV_TRY(hipMemcpyAsync(sv.walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice,
+                          sv.stream));
+  SV_TRY(hipStreamSynchronize(sv.stream));
+  sv.ntiles = (int)tiles.size();
+  sv.nwalls = (int)walls.size();
+  sv.lists_dirty = false;
+  return LBM_OK;
+}
+
+static int admit(Server& sv) {
+  while (!sv.queue.empty() && (int)sv.active.size() < sv.max_slots) {
+    PendingReq& r = sv.queue.front();
+    if (sv.in_use + r.footprint > sv.budget && !sv.active.empty()) break;  // wait for memory
+    lbm_props pr{(int32_t)r.ctype.size(), r.dens.data(), r.pres.data(), r.temp.data(),
+                 r.vx.data(),             r.vy.data(),   r.ctype.data()};
+    lbm_solver* h = nullptr;
+    int st = lbm_create(&r.params, r.nx, r.ny, r.bnd.data(), &pr, sv.precision, &h);
+    if (st == LBM_OK) st = lbm_init(h);
+    const double t = now_s();
+    if (st != LBM_OK) {
+      if (h) lbm_destroy(h);
+      lbm_request_result res{};
+      res.tag = r.tag;
+      res.status = st;
+      res.submit_s = r.submit_s;
+      res.admit_s = t;
+      res.finish_s = t;
+      res.cells = (int64_t)r.nx * r.ny;
+      sv.done.push_back(res);
+      sv.queue.pop_front();
+      continue;
+    }
+    Solver* s = reinterpret_cast<Solver*>(h);
+    sv.active.push_back(ActiveReq{s, r.nsteps, r.tag, r.submit_s, t, s->device_bytes});
+    sv.in_use += s->device_bytes;
+    sv.queue.pop_front();
+    sv.lists_dirty = true;
+  }
+  return LBM_OK;
+}
+
+static int retire(Server& sv) {
+  std::vector<ActiveReq> keep;
+  for (auto& a : sv.active) {
+    Solver* s = a.solver;
+    if (s->step < a.nsteps && !s->failed) {
+      keep.push_back(a);
+      continue;
+    }
+    lbm_request_result res{};
+    res.tag = a.tag;
+    res.status = s->failed ? LBM_FAILED : LBM_OK;
+    res.steps_done = s->step;
+    res.submit_s = a.submit_s;
+    res.admit_s = a.admit_s;
+    res.finish_s = now_s();
+    res.cells = (int64_t)s->nx * s->ny;
+    if (sv.keep) {
+      // the solver's own instance record still describes admission time
+      int su = upload_inst(*s);
+      if (su != LBM_OK) return su;
+      KeptMacro km;
+      km.nx = s->nx;
+      km.ny = s->ny;
+      const size_t n = (size_t)s->nx * s->ny;
+      km.rho.resize(n);
+      km.ux.resize(n);
+      km.uy.resize(n);
+      km.p.resize(n);
+      km.T.resize(n);
+      int st = lbm_get_macro(reinterpret_cast<lbm_solver*>(s), km.rho.data(), km.ux.data(), km.uy.data(),
+                             km.p.data(), km.T.data());
+      if (st == LBM_OK) sv.kept[a.tag] = std::move(km);
+    }
+    sv.done.push_back(res);
+    sv.in_use -= a.footprint;
+    delete s;
+    sv.lists_dirty = true;
+  }
+  sv.active.swap(keep);
+  return LBM_OK;
+}
+
+}  // namespace uqlbm
+
+using namespace uqlbm;
+
+extern "C" int lbm_server_create(int32_t precision, int32_t rk_method, size_t hbm_budget_bytes, int32_t max_slots,
+                                 lbm_server** out) {
+  if (!out || max_slots < 1 || hbm_budget_bytes == 0) return fail_with(LBM_ERR_ARG, "bad server arguments");
+  if (precision != LBM_PREC_FP64 && precision != LBM_PREC_MIXED)
+    return fail_with(LBM_ERR_ARG, "unknown precision %d", precision);
+  if (rk_method != 1) return fail_with(LBM_ERR_ARG, "RKMETHOD %d not supported by the server", rk_method);
+  if (lbm_device_count() < 1) return fail_with(LBM_ERR_NOGPU, "no gfx950 device visible");
+  std::unique_ptr<Server> sv(new Server());
+  sv->precision = precision;
+  sv->rk_method = rk_method;
+  sv->budget = hbm_budget_bytes;
+  sv->max_slots = max_slots;
+  SV_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+  SV_TRY(hipMalloc((void**)&sv->table_dev, (size_t)max_slots * sizeof(InstDev)));
+  *out = reinterpret_cast<lbm_server*>(sv.release());
+  return LBM_OK;
+}
+
+extern "C" int lbm_server_submit(lbm_server* h, const lbm_params* params, int32_t nx, int32_t ny,
+                                 const int32_t* bnd, const lbm_props* props, int32_t nsteps, int64_t tag) {
+  if (!h || !params || !bnd || !props || nx < 4 || ny < 4 || nsteps < 0 || props->num_props < 1)
+    return fail_with(LBM_ERR_ARG, "bad request");
+  Server& sv = *reinterpret_cast<Server*>(h);
+  PendingReq r;
+  r.params = *params;
+  r.nx = nx;
+  r.ny = ny;
+  r.bnd.assign(bnd, bnd + (size_t)nx * ny);
+  const int n = props->num_props;
+  r.dens.assign(props->density, props->density + n);
+  r.pres.assign(props->pressure, props->pressure + n);
+  r.temp.assign(props->temp, props->temp + n);
+  r.vx.assign(props->velocity_x, props->velocity_x + n);
+  r.vy.assign(props->velocity_y, props->velocity_y + n);
+  r.ctype.assign(props->cell_type, props->cell_type + n);
+  r.nsteps = nsteps;
+  r.tag = tag;
+  r.submit_s = now_s();
+  r.footprint = footprint_of(nx, ny, sv.precision);
+  if (r.footprint > sv.budget) return fail_with(LBM_ERR_NOMEM, "request needs %zu bytes, budget %zu", r.footprint, sv.budget);
+  sv.queue.push_back(std::move(r));
+  return LBM_OK;
+}
+
+extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* active_out, int32_t* queued_out) {
+  if (!h || tick_steps < 1) return fail_with(LBM_ERR_ARG, "bad tick arguments");
+  Server& sv = *reinterpret_cast<Server*>(h);
+  int status = LBM_OK;
+  int left = tick_steps;
+  int st = admit(sv);
+  if (st != LBM_OK) return st;
+  st = retire(sv);  // zero-step requests
+  if (st != LBM_OK) return st;
+  while (left > 0 && !sv.active.empty()) {
+    int round = left;
+    for (auto& a : sv.active) round = std::min(round, a.nsteps - a.solver->step);
+    if (round < 1) round = 1;
+    if (sv.lists_dirty) {
+      st = rebuild_lists(sv);
+      if (st != LBM_OK) return st;
+    }
+    std::vector<Solver*> members;
+    members.reserve(sv.active.size());
+    for (auto& a : sv.active) members.push_back(a.solver);
+    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls};
+    long long cells = 0;
+    for (Solver* s : members) cells += (long long)s->nx * s->ny;
+    st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round);
+    if (st < 0) return st;
+    if (st == LBM_FAILED) status = LBM_FAILED;
+    sv.lups += cells * round;
+    left -= round;
+    st = retire(sv);
+    if (st != LBM_OK) return st;
+    st = admit(sv);
+    if (st != LBM_OK) return st;
+  }
+  if (active_out) *active_out = (int32_t)sv.active.size();
+  if (queued_out) *queued_out = (int32_t)sv.queue.size();
+  return status;
+}
+
+extern "C" int lbm_server_poll(lbm_server* h, lbm_request_result* out, int32_t max) {
+  if (!h || (!out && max > 0) || max < 0) return fail_with(LBM_ERR_ARG, "bad poll arguments");
+  Server& sv = *reinterpret_cast<Server*>(h);
+  const int n = std::min<int>(max, (int)sv.done.size());
+  for (int k = 0; k < n; k++) out[k] = sv.done[k];
+  sv.done.erase(sv.done.begin(), sv.done.begin() + n);
+  return n;
+}
+
+extern "C" int lbm_server_keep_results(lbm_server* h, int32_t enable) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null server");
+  reinterpret_cast<Server*>(h)->keep = enable != 0;
+  return LBM_OK;
+}
+
+extern "C" int lbm_server_result_macro(lbm_server* h, int64_t tag, double* rho, double* ux, double* uy, double* p,
+                                       double* T) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null server");
+  Server& sv = *reinterpret_cast<Server*>(h);
+  auto it = sv.kept.find(tag);
+  if (it == sv.kept.end()) return fail_with(LBM_ERR_ARG, "no kept result for tag %lld", (long long)tag);
+  const KeptMacro& k = it->second;
+  const size_t bytes = (size_t)k.nx * k.ny * sizeof(double);
+  if (rho) std::memcpy(rho, k.rho.data(), bytes);
+  if (ux) std::memcpy(ux, k.ux.data(), bytes);
+  if (uy) std::memcpy(uy, k.uy.data(), bytes);
+  if (p) std::memcpy(p, k.p.data(), bytes);
+  if (T) std::memcpy(T, k.T.data(), bytes);
+  return LBM_OK;
+}
+
+extern "C" int lbm_server_stats(lbm_server* h, int64_t* lups_done, size_t* hbm_in_use) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null server");
+  Server& sv = *reinterpret_cast<Server*>(h);
+  if (lups_done) *lups_done = sv.lups;
+  if (hbm_in_use) *hbm_in_use = sv.in_use;
+  return LBM_OK;
+}
+
+extern "C" void lbm_server_destroy(lbm_server* h) { delete reinterpret_cast<Server*>(h); }
