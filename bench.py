@@ -44,11 +44,10 @@ def log(msg: str) -> None:
 
 def kernel_bytes_per_lup(es: int) -> dict:
     """Compulsory HBM traffic of each kernel per lattice update (values of
-    `es` bytes): stage12 reads the state and writes f2 (52 values); stage3
-    reads the state and f2 and writes f3 and G (104); final reads f3 and G
-    and writes the state (78). Halo re-reads, the cell-type map and the
-    wall-factor plane are excluded."""
-    return {"stage12": 52 * es, "stage3": 104 * es, "final": 78 * es}
+    `es` bytes): stageA reads the state and writes f3 and G (78 values);
+    stageB reads f3 and G and writes the state (78). Halo re-reads, the
+    cell-type map and the wall-closure kernels are excluded."""
+    return {"stageA": 78 * es, "stageB": 78 * es}
 
 
 def run_ensemble(members, precision: str, steps: int, warmup: int, timing: bool):
@@ -119,7 +118,7 @@ def main() -> int:
     log(f"batched mixed: {ms:.1f} ms for {args.steps} steps -> {mlups:.1f} MLUPS (status {st})")
     kb = kernel_bytes_per_lup(4)
     ktimes = t["kernels"]
-    dom = max(("stage12", "stage3", "final"), key=lambda k: ktimes.get(k, {}).get("avg_ms", 0.0))
+    dom = max(("stageA", "stageB"), key=lambda k: ktimes.get(k, {}).get("avg_ms", 0.0))
     dom_ms = ktimes[dom]["avg_ms"]
     achieved = kb[dom] * cells / (dom_ms * 1e-3) / 1e9
     step_bytes = 520.0  # 130 values per LUP in FP32 storage (fused two-stage model)
@@ -213,7 +212,7 @@ def main() -> int:
         r_mlups = rc.cells * args.riemann_steps / (rt["total_ms"] * 1e-3) / 1e6
         kb64 = kernel_bytes_per_lup(8)
         rk = rt["kernels"]
-        rdom = max(("stage12", "stage3", "final"), key=lambda k: rk.get(k, {}).get("avg_ms", 0.0))
+        rdom = max(("stageA", "stageB"), key=lambda k: rk.get(k, {}).get("avg_ms", 0.0))
         r_ach = kb64[rdom] * rc.cells / (rk[rdom]["avg_ms"] * 1e-3) / 1e9
         result["riemann_fp64"] = {
             "n": args.riemann_n, "steps": args.riemann_steps, "mlups": round(r_mlups, 2),

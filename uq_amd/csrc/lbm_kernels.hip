@@ -1,17 +1,19 @@
 // lbm_kernels.hip — gfx950 kernels of the D2Q13 thermal FD-LBM.
 //
 // One step of the reference's RKMETHOD = 1 stepper (IMEX-SSP2(3,2,2),
-// reference RK3, cpuLBM.cpp:1738-2103) is five launches, each covering every
-// tile of every instance of the group in one grid-stride (persistent) loop:
+// reference RK3, cpuLBM.cpp:1738-2103) is two tile launches, each covering
+// every tile of every instance of the group in one grid-stride (persistent)
+// loop, plus the wall closure when the group has walls:
 //
-//   k_stage12   pointwise: moments, equilibrium, implicit stages 1-2 -> f2
-//   k_alpha     wall cells: closure factor from the fluid neighbours' face
-//               fluxes of f2 (reference wallFluxes, cpuLBM.cpp:1502-1590)
-//   k_stage3    stencil: face fluxes of f2 per direction through an LDS halo
-//               tile, moments of f - dF(f2), equilibrium, implicit stage 3
-//               -> f3; partial final combination G -> the other state buffer
-//   k_alpha     same for f3
-//   k_final     stencil: face fluxes of f3 -> new state = G - dF(f3)/2
+//   [k_s12_list  f2 of the cells the wall closure's stencils read]
+//   [k_alpha     wall cells: closure factor from the fluid neighbours' face
+//                fluxes of f2 (reference wallFluxes, cpuLBM.cpp:1502-1590)]
+//   k_stageA     stages 1-3 fused: f2 of tile + halo recomputed in LDS per
+//                direction, face fluxes, moments of f - dF(f2), stage-3
+//                equilibrium -> f3; G (stage-2 half of the final
+//                combination) -> the other state buffer
+//   [k_alpha     same for f3]
+//   k_stageB     face fluxes of f3, stage-3 relaxation -> new state
 //
 // Face fluxes are staged one direction at a time: the tile's cells plus a
 // 3-cell halo cross (rows for x fluxes, columns for y fluxes) of one f and
@@ -358,19 +360,58 @@ __device__ __forceinline__ void flag_failure(const InstDev& in, int step, double
 }
 
 // ---------------------------------------------------------------------------
-// k_stage12: RK3 stages 1 and 2 (cpuLBM.cpp:1819-1924), pointwise.
+// RK3 stages 1 and 2 (cpuLBM.cpp:1819-1924) are pointwise. Their per-cell
+// coefficients live in a record; positions reached through a mirror store the
+// mirrored velocity, since eq_I(u -> -u) == eq_{mirror_ew(I)}(u) and likewise
+// for v, so one direction I serves every halo position.
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void stage12_cell(const InstDev& in, const TS* src, TS* dst, long long o,
-                                             int step, Seq<Is...>) {
-  const long long P = in.plane;
-  double f[kQ], h[kQ];
-  double rho = 0, rux = 0, ruy = 0, sh = 0;
+template <typename TC>
+struct Rec12 {
+  EqIn<TC> q;
+  TC cU, cV;          // c*U, c*V: e_i . u = lat_x(i)*cU + lat_y(i)*cV
+  TC af, ah, ahf, g;  // dt/(2 tf), dt/(2 th), dt/(2 thf), 1/thf
+};
+
+template <typename TC>
+__device__ __forceinline__ Rec12<TC> make_rec12(const InstDev& in, Macro m, bool flip_x, bool flip_y) {
+  if (flip_x) m.ux = -m.ux;
+  if (flip_y) m.uy = -m.uy;
+  Rec12<TC> r;
+  r.q = eq_inputs<TC>(in, m);
+  r.cU = TC(in.c) * TC(m.ux);
+  r.cV = TC(in.c) * TC(m.uy);
+  const TC dt = TC(in.dt);
+  r.af = dt / (TC(2) * TC(m.tf));
+  r.ah = dt / (TC(2) * TC(m.th));
+  r.ahf = dt / (TC(2) * TC(m.thf));
+  r.g = TC(1) / TC(m.thf);
+  return r;
+}
+
+// f2, h2 of direction I from the raw f, h and the cell record; also returns
+// the stage-1 equilibria (feq2 = feq1, reference cpuLBM.cpp:1874-1875).
+template <int I, typename TC>
+__device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC& fe, TC& he, TC& f2, TC& h2) {
+  eq_dir<I>(r.q, fe, he);
+  const TC f1 = (fv + r.af * fe) / (TC(1) + r.af);
+  const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
+  const TC h1 = (hv - r.ahf * eu * (fe - f1) + r.ah * he) / (TC(1) + r.ah);
+  f2 = (fv - r.af * (fe - f1) + r.af * fe) / (TC(1) + r.af);
+  h2 = (hv - hdt * ((eu * r.g) * (fe - f2) - (eu * r.g) * (fe - f1)) - r.ah * (he - h1) + r.ah * he) /
+       (TC(1) + r.ah);
+}
+
+template <typename TS>
+__device__ __forceinline__ void load26(const TS* src, long long P, long long o, double* f, double* h) {
 #pragma unroll
   for (int i = 0; i < kQ; i++) {
     f[i] = double(src[P * i + o]);
     h[i] = double(src[P * (kQ + i) + o]);
   }
+}
+
+__device__ __forceinline__ void moments26(const InstDev& in, const double* f, const double* h, Macro& m) {
+  double rho = 0, rux = 0, ruy = 0, sh = 0;
 #pragma unroll
   for (int i = 0; i < kQ; i++) {
     rho += f[i];
@@ -378,169 +419,202 @@ __device__ __forceinline__ void stage12_cell(const InstDev& in, const TS* src, T
     ruy += f[i] * (lat_y(i) * in.c);
     sh += h[i];
   }
-  Macro m;
   finish_macro(in, rho, rux, ruy, sh, m);
-  const EqIn<TC> q = eq_inputs<TC>(in, m);
-  const TC dt = TC(in.dt);
-  const TC af = dt / (TC(2) * TC(m.tf));
-  const TC ah = dt / (TC(2) * TC(m.th));
-  const TC ahf = dt / (TC(2) * TC(m.thf));
-  const TC hdt = dt / TC(2);
-  const TC g = TC(1) / TC(m.thf);
-  const TC U = TC(m.ux), V = TC(m.uy), c = TC(in.c);
-  double sumfe = 0;
+}
+
+// Halo positions of a tile: the x band (H columns either side, tile rows)
+// then the y band (H rows either side, tile columns).
+constexpr int kNXB = 2 * kH * kTY, kNYB = 2 * kH * kTX, kNHalo = kNXB + kNYB;
+
+__device__ __forceinline__ void halo_pos(int k, int& lx, int& ly) {
+  if (k < kNXB) {
+    const int side = k / (kH * kTY), kk = k % (kH * kTY);
+    ly = kk / kH;
+    lx = side ? kTX + kk % kH : -kH + kk % kH;
+  } else {
+    const int k2 = k - kNXB, side = k2 / (kH * kTX), kk = k2 % (kH * kTX);
+    lx = kk % kTX;
+    ly = side ? kTY + kk / kTX : -kH + kk / kTX;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_s12_list: f2 for a list of cells (the stencil cells of the wall closure).
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int... Is>
+__device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, TS* dst, long long o, Seq<Is...>) {
+  const long long P = in.plane;
+  double f[kQ], h[kQ];
+  load26(src, P, o, f, h);
+  Macro m;
+  moments26(in, f, h, m);
+  const Rec12<TC> r = make_rec12<TC>(in, m, false, false);
+  const TC hdt = TC(in.dt) / TC(2);
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
-    TC fe, he;
-    eq_dir<I>(q, fe, he);
-    sumfe += double(fe);
-    const TC fv = TC(f[I]), hv = TC(h[I]);
-    const TC f1 = (fv + af * fe) / (TC(1) + af);
-    const TC eu = TC(lat_x(I)) * c * U + TC(lat_y(I)) * c * V;
-    const TC h1 = (hv - ahf * eu * (fe - f1) + ah * he) / (TC(1) + ah);
-    const TC f2 = (fv - af * (fe - f1) + af * fe) / (TC(1) + af);
-    const TC h2 = (hv - hdt * ((eu * g) * (fe - f2) - (eu * g) * (fe - f1)) - ah * (he - h1) + ah * he) /
-                  (TC(1) + ah);
+    TC fe, he, f2, h2;
+    s12<I>(r, hdt, TC(f[I]), TC(h[I]), fe, he, f2, h2);
     dst[P * I + o] = TS(f2);
     dst[P * (kQ + I) + o] = TS(h2);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m.rho, sumfe);
 }
 
 template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) void k_stage12(const InstDev* insts, const TileRef* tiles, int ntiles,
-                                                 int par, int step) {
-  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
-    const TileRef tr = tiles[t];
-    const InstDev& in = insts[tr.inst];
-    if (frozen_by_failure(in, step)) continue;
-    const int x = tr.tx * kTX + (int)threadIdx.x % kTX;
-    const int y = tr.ty * kTY + (int)threadIdx.x / kTX;
-    if (x >= in.nx || y >= in.ny) continue;
-    const long long o = (long long)y * in.pitch + x;
-    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
-    TS* dst = reinterpret_cast<TS*>(in.f2);
-    if (in.ctype[o] > 0) {
+__global__ __launch_bounds__(256) void k_s12_list(const InstDev* insts, const WallRef* cells, int ncells, int par,
+                                                   int step) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncells) return;
+  const WallRef cr = cells[k];
+  const InstDev& in = insts[cr.inst];
+  if (frozen_by_failure(in, step)) return;
+  const long long o = (long long)cr.y * in.pitch + cr.x;
+  const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+  TS* dst = reinterpret_cast<TS*>(in.f2);
+  if (in.ctype[o] > 0) {
 #pragma unroll
-      for (int k = 0; k < kPlanes; k++) dst[in.plane * k + o] = src[in.plane * k + o];
-      continue;
-    }
-    stage12_cell<TS, TC>(in, src, dst, o, step, MakeSeq<kQ>::type{});
+    for (int q = 0; q < kPlanes; q++) dst[in.plane * q + o] = src[in.plane * q + o];
+    return;
   }
+  s12_cell<TS, TC>(in, src, dst, o, MakeSeq<kQ>::type{});
 }
 
 // ---------------------------------------------------------------------------
-// k_stage3: RK3 stage 3 plus the stage-2 half of the final combination
-// (cpuLBM.cpp:1926-1993, 2009-2040).
+// k_stageA: RK3 stages 1-3 fused (cpuLBM.cpp:1819-1993) plus the stage-2
+// half of the final combination (:2022, :2039).
+//
+// Per tile: stage-1/2 records for the halo cross go to LDS; per direction
+// the f2/h2 plane of tile + halo is computed into the staging buffer, face
+// fluxes follow, and each cell keeps two values (P = f - dF2 + af2 (feq2 -
+// f2), Q likewise for h) while G = f - dF2/2 + dt/2 (feq2 - f2)/tf2 (and the
+// h analogue) goes straight to the output state buffer. After the last
+// direction the moments of f - dF(f2) give the stage-3 equilibrium and f3.
 // ---------------------------------------------------------------------------
+template <typename TC>
+struct HaloStore {
+  Rec12<TC> rec[kNHalo];
+};
+
 template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void stage3_tile(Stage<TC>& S, const Region& R, const InstDev& in,
+__device__ __forceinline__ void stageA_tile(Stage<TC>& S, HaloStore<TC>& HS, const Region& R, const InstDev& in,
                                             const TileRef& tr, int par, int step, Seq<Is...>) {
   const int lx = (int)threadIdx.x % kTX, ly = (int)threadIdx.x / kTX;
   const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
-  const bool active = x < in.nx && y < in.ny;
-  const long long P = in.plane;
-  const long long o = active ? (long long)y * in.pitch + x : 0;
-  const int type = active ? in.ctype[o] : 2;
-  const bool fluid = active && type == 0;
+  const bool own_in = x < in.nx && y < in.ny;
+  const unsigned mo = R.meta[ly + kH][lx + kH];
+  const long long oo = R.off[ly + kH][lx + kH];   // mapped physical cell of the own position
+  const int type_o = mo & kMetaType;
+  const bool fluid = own_in && type_o == 0;
   const bool solids = tr.flags & kTileSolid;
-  void* cur = in.state[in.parity0 ^ par];
+  const long long P = in.plane;
+  const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
   TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
   TS* f3buf = reinterpret_cast<TS*>(in.f3);
-  const TS* src = reinterpret_cast<const TS*>(cur);
-
-  // own raw state and stage-1/2 moments
-  TC f[kQ], h[kQ];
-  double rho = 1, rux = 0, ruy = 0, sh = 1;
-  if (fluid) {
-    rho = 0;
-    sh = 0;
-#pragma unroll
-    for (int i = 0; i < kQ; i++) {
-      const double fi = double(src[P * i + o]);
-      const double hi = double(src[P * (kQ + i) + o]);
-      f[i] = TC(fi);
-      h[i] = TC(hi);
-      rho += fi;
-      rux += fi * (lat_x(i) * in.c);
-      ruy += fi * (lat_y(i) * in.c);
-      sh += hi;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < kQ; i++) { f[i] = TC(1); h[i] = TC(1); }
-  }
-  Macro m1;
-  finish_macro(in, rho, rux, ruy, sh, m1);
-  const EqIn<TC> q1 = eq_inputs<TC>(in, m1);
   const TC dt = TC(in.dt), hdt = dt / TC(2);
+
+  // --- phase 1: own record (registers) and halo records (LDS) -------------
+  double fr[kQ], hr[kQ];
+  load26(src, P, oo, fr, hr);
+  Macro m1;
+  moments26(in, fr, hr, m1);
+  const Rec12<TC> ro = make_rec12<TC>(in, m1, mo & kMetaMirEW, mo & kMetaMirNS);
+  for (int k = threadIdx.x; k < kNHalo; k += kNT) {
+    int hx, hy;
+    halo_pos(k, hx, hy);
+    const unsigned mh = R.meta[hy + kH][hx + kH];
+    if ((mh & kMetaType) != 0) continue;
+    double f[kQ], h[kQ];
+    load26(src, P, R.off[hy + kH][hx + kH], f, h);
+    Macro m;
+    moments26(in, f, h, m);
+    HS.rec[k] = make_rec12<TC>(in, m, mh & kMetaMirEW, mh & kMetaMirNS);
+  }
+  __syncthreads();
+
+  TC Pf[kQ], Qh[kQ];
+  double rho3 = 0, rux3 = 0, ruy3 = 0, sh3 = 0;
   const TC tf2 = TC(m1.tf), th2 = TC(m1.th), thf2 = TC(m1.thf);
   const TC af2 = dt / (TC(2) * tf2), ah2 = dt / (TC(2) * th2);
-  const TC U1 = TC(m1.ux), V1 = TC(m1.uy), c = TC(in.c);
-
-  TC Pf[kQ], Gf[kQ], Qh[kQ], Gh[kQ];
-  double rho3 = 0, rux3 = 0, ruy3 = 0, sh3 = 0;
-
-  auto combine2 = [&](auto Ic, TC dF, TC dH, TC f2, TC h2) {
-    constexpr int I = decltype(Ic)::value;
-    TC fe, he;
-    eq_dir<I>(q1, fe, he);
-    const TC D = fe - f2;
-    const TC eu2 = TC(lat_x(I)) * c * U1 + TC(lat_y(I)) * c * V1;
-    const TC A2 = (eu2 / thf2) * D;
-    Pf[I] = (f[I] - dF) + af2 * D;
-    Gf[I] = (f[I] - TC(0.5) * dF) + hdt * (D / tf2);
-    Qh[I] = ((h[I] - dH) - hdt * A2) + ah2 * (he - h2);
-    Gh[I] = ((h[I] - TC(0.5) * dH) + hdt * ((he - h2) / th2)) - hdt * A2;
-    const double fd = double(f[I]), dd = double(dF);
-    const double exc = lat_x(I) * in.c, eyc = lat_y(I) * in.c;
-    rho3 += fd - dd;
-    rux3 += fd * exc - dd * exc;
-    ruy3 += fd * eyc - dd * eyc;
-    sh3 += double(h[I]) - double(dH);
-  };
 
   auto one_dir = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
-    if constexpr (I == 0) {
-      TC f2 = TC(0), h2 = TC(0);
-      if (fluid) {
-        f2 = TC(plane<TS>(in.f2, P, 0)[o]);
-        h2 = TC(plane<TS>(in.f2, P, kQ)[o]);
+    constexpr int EX = lat_x(I), EY = lat_y(I);
+    // own position: f2/h2 from the record (raw value of the mirrored direction
+    // when the position lies outside the grid)
+    TC fe = 0, he = 0, f2 = 0, h2 = 0;
+    {
+      const int d = remap_dir(I, mo);
+      const TC fv = own_in ? TC(fr[I]) : TC(plane<TS>((void*)src, P, d)[oo]);
+      const TC hv = own_in ? TC(hr[I]) : TC(plane<TS>((void*)src, P, kQ + d)[oo]);
+      if (type_o == 0) {
+        s12<I>(ro, hdt, fv, hv, fe, he, f2, h2);
+      } else {
+        f2 = fv;
+        h2 = hv;
       }
-      combine2(Ic, TC(0), TC(0), f2, h2);
-    } else {
-      stage_raw<I, TS, TC>(S, R, in, in.f2);
+    }
+    if constexpr (I != 0) {
+      S.f[ly + kH][lx + kH] = f2;
+      S.h[ly + kH][lx + kH] = h2;
+      constexpr int k0 = EX ? 0 : kNXB;
+      constexpr int k1 = EY ? kNHalo : kNXB;
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += kNT) {
+        int hx, hy;
+        halo_pos(k, hx, hy);
+        const unsigned mh = R.meta[hy + kH][hx + kH];
+        const long long oh = R.off[hy + kH][hx + kH];
+        const int d = remap_dir(I, mh);
+        const TC fv = TC(plane<TS>((void*)src, P, d)[oh]);
+        const TC hv = TC(plane<TS>((void*)src, P, kQ + d)[oh]);
+        TC a = fv, b = hv;
+        if ((mh & kMetaType) == 0) {
+          TC fe_, he_;
+          s12<I>(HS.rec[k], hdt, fv, hv, fe_, he_, a, b);
+        }
+        S.f[hy + kH][hx + kH] = a;
+        S.h[hy + kH][hx + kH] = b;
+      }
       __syncthreads();
       tile_faces<I, TC>(S, R, in, solids);
       __syncthreads();
-      TC dF = TC(0), dH = TC(0);
-      if (fluid) own_net_flux<I, TC>(S, R, in, lx, ly, dF, dH);
-      const TC f2 = S.f[ly + kH][lx + kH], h2 = S.h[ly + kH][lx + kH];
-      combine2(Ic, dF, dH, f2, h2);
-      __syncthreads();
+    }
+    if (fluid) {
+      TC dF = 0, dH = 0;
+      if constexpr (I != 0) own_net_flux<I, TC>(S, R, in, lx, ly, dF, dH);
+      const TC fv = TC(fr[I]), hv = TC(hr[I]);
+      const TC D = fe - f2;
+      const TC eu2 = TC(lat_x(I)) * ro.cU + TC(lat_y(I)) * ro.cV;
+      const TC A2 = (eu2 / thf2) * D;
+      Pf[I] = (fv - dF) + af2 * D;
+      Qh[I] = ((hv - dH) - hdt * A2) + ah2 * (he - h2);
+      const TC gf = (fv - TC(0.5) * dF) + hdt * (D / tf2);
+      const TC gh = ((hv - TC(0.5) * dH) + hdt * ((he - h2) / th2)) - hdt * A2;
+      out[P * I + oo] = TS(gf);
+      out[P * (kQ + I) + oo] = TS(gh);
+      const double fd = fr[I], dd = double(dF);
+      const double exc = lat_x(I) * in.c, eyc = lat_y(I) * in.c;
+      rho3 += fd - dd;
+      rux3 += fd * exc - dd * exc;
+      ruy3 += fd * eyc - dd * eyc;
+      sh3 += hr[I] - double(dH);
+    } else if (own_in) {
+      // frozen cell: f3 = f2 = f and the new state is f
+      const TS a = src[P * I + oo], b = src[P * (kQ + I) + oo];
+      out[P * I + oo] = a;
+      out[P * (kQ + I) + oo] = b;
+      f3buf[P * I + oo] = a;
+      f3buf[P * (kQ + I) + oo] = b;
     }
   };
   (one_dir(std::integral_constant<int, Is>{}), ...);
 
-  if (!active) return;
-  if (!fluid) {
-    // frozen cell: f3 = f2 = f and the final state is f
-#pragma unroll
-    for (int k = 0; k < kPlanes; k++) {
-      const TS v = src[P * k + o];
-      f3buf[P * k + o] = v;
-      out[P * k + o] = v;
-    }
-    return;
-  }
+  if (!fluid) return;
   Macro m3;
   finish_macro(in, rho3, rux3, ruy3, sh3, m3);
   const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
   const TC tf3 = TC(m3.tf), th3 = TC(m3.th), thf3 = TC(m3.thf);
   const TC af3 = dt / (TC(2) * tf3), ah3 = dt / (TC(2) * th3);
-  const TC U3 = TC(m3.ux), V3 = TC(m3.uy);
+  const TC cU3 = TC(in.c) * TC(m3.ux), cV3 = TC(in.c) * TC(m3.uy);
   double sumfe = 0;
   auto finish = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
@@ -548,22 +622,89 @@ __device__ __forceinline__ void stage3_tile(Stage<TC>& S, const Region& R, const
     eq_dir<I>(q3, fe, he);
     sumfe += double(fe);
     const TC f3 = (Pf[I] + af3 * fe) / (TC(1) + af3);
-    const TC eu3 = TC(lat_x(I)) * c * U3 + TC(lat_y(I)) * c * V3;
+    const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
     const TC A3 = (eu3 / thf3) * (fe - f3);
     const TC h3 = ((Qh[I] - hdt * A3) + ah3 * he) / (TC(1) + ah3);
-    const TC gf = Gf[I] + hdt * ((fe - f3) / tf3);
-    const TC gh = (Gh[I] + hdt * ((he - h3) / th3)) - hdt * A3;
-    f3buf[P * I + o] = TS(f3);
-    f3buf[P * (kQ + I) + o] = TS(h3);
-    out[P * I + o] = TS(gf);
-    out[P * (kQ + I) + o] = TS(gh);
+    f3buf[P * I + oo] = TS(f3);
+    f3buf[P * (kQ + I) + oo] = TS(h3);
   };
   (finish(std::integral_constant<int, Is>{}), ...);
   flag_failure(in, step, m3.rho, sumfe);
 }
 
 template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) void k_stage3(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+__global__ __launch_bounds__(kNT) void k_stageA(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                                int step) {
+  __shared__ Region R;
+  __shared__ Stage<TC> S;
+  __shared__ HaloStore<TC> HS;
+  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const InstDev& in = insts[tr.inst];
+    if (frozen_by_failure(in, step)) continue;
+    __syncthreads();
+    setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
+    __syncthreads();
+    stageA_tile<TS, TC>(S, HS, R, in, tr, par, step, MakeSeq<kQ>::type{});
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_stageB: the rest of the final combination (cpuLBM.cpp:1995-2042): the
+// stage-3 relaxation terms (equilibrium from the moments of f3, which equal
+// the moments of f - dF(f2)) and -dF(f3)/2.
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int... Is>
+__device__ __forceinline__ void stageB_tile(Stage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
+                                            int par, Seq<Is...>) {
+  const int lx = (int)threadIdx.x % kTX, ly = (int)threadIdx.x / kTX;
+  const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
+  const bool own_in = x < in.nx && y < in.ny;
+  const long long oo = R.off[ly + kH][lx + kH];
+  const bool fluid = own_in && (R.meta[ly + kH][lx + kH] & kMetaType) == 0;
+  const bool solids = tr.flags & kTileSolid;
+  const long long P = in.plane;
+  const TS* f3b = reinterpret_cast<const TS*>(in.f3);
+  TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+  const TC dt = TC(in.dt), hdt = dt / TC(2);
+  Macro m3{};
+  m3.rho = 1;
+  m3.tf = m3.th = m3.thf = 1;
+  if (fluid) {
+    double f[kQ], h[kQ];
+    load26(f3b, P, oo, f, h);
+    moments26(in, f, h, m3);
+  }
+  const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
+  const TC tf3 = TC(m3.tf), th3 = TC(m3.th), thf3 = TC(m3.thf);
+  const TC cU3 = TC(in.c) * TC(m3.ux), cV3 = TC(in.c) * TC(m3.uy);
+  auto one_dir = [&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    TC dF = 0, dH = 0;
+    if constexpr (I != 0) {
+      stage_raw<I, TS, TC>(S, R, in, in.f3);
+      __syncthreads();
+      tile_faces<I, TC>(S, R, in, solids);
+      __syncthreads();
+      if (fluid) own_net_flux<I, TC>(S, R, in, lx, ly, dF, dH);
+    }
+    if (fluid) {
+      TC fe, he;
+      eq_dir<I>(q3, fe, he);
+      const TC f3 = TC(f3b[P * I + oo]), h3 = TC(f3b[P * (kQ + I) + oo]);
+      const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
+      const TC A3 = (eu3 / thf3) * (fe - f3);
+      TS* pf = plane<TS>(out, P, I);
+      TS* ph = plane<TS>(out, P, kQ + I);
+      pf[oo] = TS((TC(pf[oo]) + hdt * ((fe - f3) / tf3)) - TC(0.5) * dF);
+      ph[oo] = TS(((TC(ph[oo]) + hdt * ((he - h3) / th3)) - hdt * A3) - TC(0.5) * dH);
+    }
+  };
+  (one_dir(std::integral_constant<int, Is>{}), ...);
+}
+
+template <typename TS, typename TC>
+__global__ __launch_bounds__(kNT) void k_stageB(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
                                                 int step) {
   __shared__ Region R;
   __shared__ Stage<TC> S;
@@ -574,58 +715,7 @@ __global__ __launch_bounds__(kNT) void k_stage3(const InstDev* insts, const Tile
     __syncthreads();
     setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
     __syncthreads();
-    stage3_tile<TS, TC>(S, R, in, tr, par, step, MakeSeq<kQ>::type{});
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_final: new state = G - dF(f3)/2 (cpuLBM.cpp:1995-2042).
-// ---------------------------------------------------------------------------
-template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void final_tile(Stage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
-                                           int par, Seq<Is...>) {
-  const int lx = (int)threadIdx.x % kTX, ly = (int)threadIdx.x / kTX;
-  const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
-  const bool active = x < in.nx && y < in.ny;
-  const long long P = in.plane;
-  const long long o = active ? (long long)y * in.pitch + x : 0;
-  const bool fluid = active && in.ctype[o] == 0;
-  const bool solids = tr.flags & kTileSolid;
-  TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
-  auto one_dir = [&](auto Ic) {
-    constexpr int I = decltype(Ic)::value;
-    if constexpr (I != 0) {
-      stage_raw<I, TS, TC>(S, R, in, in.f3);
-      __syncthreads();
-      tile_faces<I, TC>(S, R, in, solids);
-      __syncthreads();
-      if (fluid) {
-        TC dF, dH;
-        own_net_flux<I, TC>(S, R, in, lx, ly, dF, dH);
-        TS* pf = plane<TS>(out, P, I);
-        TS* ph = plane<TS>(out, P, kQ + I);
-        pf[o] = TS(TC(pf[o]) - TC(0.5) * dF);
-        ph[o] = TS(TC(ph[o]) - TC(0.5) * dH);
-      }
-      __syncthreads();
-    }
-  };
-  (one_dir(std::integral_constant<int, Is>{}), ...);
-}
-
-template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) void k_final(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                               int step) {
-  __shared__ Region R;
-  __shared__ Stage<TC> S;
-  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
-    const TileRef tr = tiles[t];
-    const InstDev& in = insts[tr.inst];
-    if (frozen_by_failure(in, step)) continue;
-    __syncthreads();
-    setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
-    __syncthreads();
-    final_tile<TS, TC>(S, R, in, tr, par, MakeSeq<kQ>::type{});
+    stageB_tile<TS, TC>(S, R, in, tr, par, MakeSeq<kQ>::type{});
   }
 }
 
@@ -812,30 +902,28 @@ static int grid_for(int ntiles, const void* kernel) {
 
 template <typename TS, typename TC>
 void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
-  static int g12 = 0, g3 = 0, gf = 0;
-  if (!g12) g12 = grid_for<TS, TC>(1 << 30, (const void*)k_stage12<TS, TC>);
-  if (!g3) g3 = grid_for<TS, TC>(1 << 30, (const void*)k_stage3<TS, TC>);
-  if (!gf) gf = grid_for<TS, TC>(1 << 30, (const void*)k_final<TS, TC>);
+  static int gA = 0, gB = 0;
+  if (!gA) gA = grid_for<TS, TC>(1 << 30, (const void*)k_stageA<TS, TC>);
+  if (!gB) gB = grid_for<TS, TC>(1 << 30, (const void*)k_stageB<TS, TC>);
   const int n = L.ntiles;
-  const int walls_blocks = (L.nwalls + 255) / 256;
-  if (tm) tm->begin(0, s);
-  k_stage12<TS, TC><<<n < g12 ? n : g12, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
-  if (tm) tm->end(0, s);
   if (L.nwalls) {
+    if (tm) tm->begin(0, s);
+    k_s12_list<TS, TC><<<(L.ncells + 255) / 256, 256, 0, s>>>(L.insts, L.cells, L.ncells, par, step);
+    if (tm) tm->end(0, s);
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<walls_blocks, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, step);
+    k_alpha<TS, TC><<<(L.nwalls + 255) / 256, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  k_stage3<TS, TC><<<n < g3 ? n : g3, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  k_stageA<TS, TC><<<n < gA ? n : gA, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<walls_blocks, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, step);
+    k_alpha<TS, TC><<<(L.nwalls + 255) / 256, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(3, s);
-  k_final<TS, TC><<<n < gf ? n : gf, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  k_stageB<TS, TC><<<n < gB ? n : gB, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(3, s);
 }
 

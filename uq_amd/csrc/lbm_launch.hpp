@@ -19,6 +19,8 @@ struct StepLaunch {
   int ntiles = 0;
   const WallRef* walls = nullptr;
   int nwalls = 0;
+  const WallRef* cells = nullptr;  // cells whose f2 the wall closure reads
+  int ncells = 0;
 };
 
 // Optional per-kernel timing with HIP events recorded on the launch stream

@@ -282,7 +282,7 @@ static size_t elem_size(int precision) { return precision == LBM_PREC_FP64 ? siz
 
 Solver::~Solver() {
   for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
-                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev})
+                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)cells_dev, (void*)macro_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -325,7 +325,8 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.Pr = s.p.Pr;
 }
 
-static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls, int inst) {
+static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls,
+                        std::vector<WallRef>& cells, int inst) {
   const int TW = tile_w(), TH = tile_h(), H = tile_halo();
   const int tx_n = (s.nx + TW - 1) / TW, ty_n = (s.ny + TH - 1) / TH;
   // prefix count of solid cells for the per-tile solid flag
@@ -366,6 +367,31 @@ static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vecto
       }
       if (need) walls.push_back(WallRef{inst, x, y});
     }
+  // cells the closure reads: each wall cell and the 5-point stencil lines
+  // through its non-solid axis neighbours
+  std::vector<char> mark((size_t)s.nx * s.ny, 0);
+  auto add = [&](int x, int y) {
+    const size_t k = (size_t)y * s.nx + x;
+    if (!mark[k]) {
+      mark[k] = 1;
+      cells.push_back(WallRef{inst, x, y});
+    }
+  };
+  static const int dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};
+  for (const WallRef& w : walls) {
+    add(w.x, w.y);
+    for (int j = 0; j < 4; j++) {
+      int m;
+      bool cl;
+      const int xn = ref_map(s, w.x + dxs[j], 0, m, cl);
+      const int yn = ref_map(s, w.y + dys[j], 1, m, cl);
+      if (type_at(s, xn, yn) >= 2) continue;
+      for (int k = -2; k <= 2; k++) {
+        if (j < 2) add(ref_map(s, xn + k, 0, m, cl), yn);
+        else add(xn, ref_map(s, yn + k, 1, m, cl));
+      }
+    }
+  }
 }
 
 }  // namespace uqlbm
@@ -465,18 +491,24 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   for (void* b : {s->state[0], s->state[1], s->f2, s->f3}) HIP_TRY(hipMemset(b, 0, plane * kPlanes * es));
 
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls;
-  build_lists(*s, tiles, walls, 0);
+  std::vector<WallRef> walls, cells;
+  build_lists(*s, tiles, walls, cells, 0);
   s->ntiles = (int)tiles.size();
   s->nwalls = (int)walls.size();
+  s->ncells = (int)cells.size();
   HIP_TRY(hipMalloc((void**)&s->tiles_dev, tiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(s->tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   if (!walls.empty()) {
     HIP_TRY(hipMalloc((void**)&s->walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(s->walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
+  if (!cells.empty()) {
+    HIP_TRY(hipMalloc((void**)&s->cells_dev, cells.size() * sizeof(WallRef)));
+    HIP_TRY(hipMemcpy(s->cells_dev, cells.data(), cells.size() * sizeof(WallRef), hipMemcpyHostToDevice));
+  }
   s->tiles_host = std::move(tiles);
   s->walls_host = std::move(walls);
+  s->cells_host = std::move(cells);
   st = upload_inst(*s);
   if (st != LBM_OK) return st;
   *out = reinterpret_cast<lbm_solver*>(s.release());
@@ -534,7 +566,7 @@ static KernelTimer* active_timer() {
   if (!g_timing_enabled) return nullptr;
   if (!g_timer) {
     g_timer = new KernelTimer();
-    g_timer->set_names({"stage12", "alpha", "stage3", "final"});
+    g_timer->set_names({"s12_walls", "alpha", "stageA", "stageB"});
   }
   return g_timer;
 }
@@ -604,7 +636,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
   if (s.failed) return LBM_FAILED;
   if (nsteps == 0) return LBM_OK;
-  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls};
+  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls, s.cells_dev, s.ncells};
   std::vector<Solver*> one{&s};
   int st = LBM_OK;
   if (want_residual && nsteps > 1) {
@@ -768,7 +800,7 @@ extern "C" int lbm_batch_create(lbm_solver* const* members, int32_t n, lbm_batch
 
 int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls;
+  std::vector<WallRef> walls, cells;
   for (size_t k = 0; k < b.members.size(); k++) {
     for (TileRef t : b.members[k]->tiles_host) {
       t.inst = (int)k;
@@ -778,9 +810,14 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
       w.inst = (int)k;
       walls.push_back(w);
     }
+    for (WallRef w : b.members[k]->cells_host) {
+      w.inst = (int)k;
+      cells.push_back(w);
+    }
   }
   b.ntiles = (int)tiles.size();
   b.nwalls = (int)walls.size();
+  b.ncells = (int)cells.size();
   HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
@@ -789,22 +826,22 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
     HIP_TRY(hipMalloc((void**)&b.walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(b.walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
+  if (!cells.empty()) {
+    HIP_TRY(hipMalloc((void**)&b.cells_dev, cells.size() * sizeof(WallRef)));
+    HIP_TRY(hipMemcpy(b.cells_dev, cells.data(), cells.size() * sizeof(WallRef), hipMemcpyHostToDevice));
+  }
   *out = reinterpret_cast<lbm_batch*>(new Batch(std::move(b)));
-  b.inst_dev = nullptr;
-  b.tiles_dev = nullptr;
-  b.walls_dev = nullptr;
-  b.stream = nullptr;
   return LBM_OK;
 }
 
 uqlbm::Batch::~Batch() {
-  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev})
+  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)cells_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 int uqlbm::batch_step(Batch& b, int nsteps) {
-  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls};
+  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls, b.cells_dev, b.ncells};
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

@@ -31,6 +31,7 @@ struct Solver {
   std::vector<uint8_t> ctype_host;  // [y][nx]
   std::vector<TileRef> tiles_host;
   std::vector<WallRef> walls_host;
+  std::vector<WallRef> cells_host;  // stencil cells of the wall closure
 
   hipStream_t stream = nullptr;
   void* state[2] = {nullptr, nullptr};
@@ -45,8 +46,9 @@ struct Solver {
   InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
+  WallRef* cells_dev = nullptr;
   double* macro_dev = nullptr;
-  int ntiles = 0, nwalls = 0;
+  int ntiles = 0, nwalls = 0, ncells = 0;
   size_t device_bytes = 0;
 
   int cur = 0;    // state buffer holding the current state
@@ -67,7 +69,8 @@ struct Batch {
   InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  int ntiles = 0, nwalls = 0;
+  WallRef* cells_dev = nullptr;
+  int ntiles = 0, nwalls = 0, ncells = 0;
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -78,12 +81,15 @@ struct Batch {
     inst_dev = o.inst_dev;
     tiles_dev = o.tiles_dev;
     walls_dev = o.walls_dev;
+    cells_dev = o.cells_dev;
     ntiles = o.ntiles;
     nwalls = o.nwalls;
+    ncells = o.ncells;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;
     o.walls_dev = nullptr;
+    o.cells_dev = nullptr;
     return *this;
   }
   ~Batch();

@@ -61,8 +61,9 @@ struct Server {
   InstDev* table_dev = nullptr;
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  size_t tiles_cap = 0, walls_cap = 0;
-  int ntiles = 0, nwalls = 0;
+  WallRef* cells_dev = nullptr;
+  size_t tiles_cap = 0, walls_cap = 0, cells_cap = 0;
+  int ntiles = 0, nwalls = 0, ncells = 0;
   bool lists_dirty = true;
   ~Server();
 };
@@ -82,7 +83,7 @@ static size_t footprint_of(int nx, int ny, int precision) {
 
 Server::~Server() {
   for (auto& a : active) delete a.solver;
-  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev})
+  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)cells_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -95,7 +96,7 @@ Server::~Server() {
 
 static int rebuild_lists(Server& sv) {
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls;
+  std::vector<WallRef> walls, cells;
   for (size_t k = 0; k < sv.active.size(); k++) {
     for (TileRef t : sv.active[k].solver->tiles_host) {
       t.inst = (int)k;
@@ -104,6 +105,10 @@ static int rebuild_lists(Server& sv) {
     for (WallRef w : sv.active[k].solver->walls_host) {
       w.inst = (int)k;
       walls.push_back(w);
+    }
+    for (WallRef w : sv.active[k].solver->cells_host) {
+      w.inst = (int)k;
+      cells.push_back(w);
     }
   }
   if (tiles.size() > sv.tiles_cap) {
@@ -116,6 +121,14 @@ static int rebuild_lists(Server& sv) {
     sv.walls_cap = std::max(walls.size(), sv.walls_cap * 2);
     SV_TRY(hipMalloc((void**)&sv.walls_dev, sv.walls_cap * sizeof(WallRef)));
   }
+  if (cells.size() > sv.cells_cap) {
+    if (sv.cells_dev) SV_TRY(hipFree(sv.cells_dev));
+    sv.cells_cap = std::max(cells.size(), sv.cells_cap * 2);
+    SV_TRY(hipMalloc((void**)&sv.cells_dev, sv.cells_cap * sizeof(WallRef)));
+  }
+  if (!cells.empty())
+    SV_TRY(hipMemcpyAsync(sv.cells_dev, cells.data(), cells.size() * sizeof(WallRef), hipMemcpyHostToDevice,
+                          sv.stream));
   if (!tiles.empty())
     SV_TRY(hipMemcpyAsync(sv.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
                           sv.stream));
@@ -125,6 +138,7 @@ static int rebuild_lists(Server& sv) {
   SV_TRY(hipStreamSynchronize(sv.stream));
   sv.ntiles = (int)tiles.size();
   sv.nwalls = (int)walls.size();
+  sv.ncells = (int)cells.size();
   sv.lists_dirty = false;
   return LBM_OK;
 }
@@ -271,7 +285,7 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     std::vector<Solver*> members;
     members.reserve(sv.active.size());
     for (auto& a : sv.active) members.push_back(a.solver);
-    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls};
+    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls, sv.cells_dev, sv.ncells};
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
     st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round);
