@@ -44,10 +44,12 @@ def log(msg: str) -> None:
 
 def kernel_bytes_per_lup(es: int) -> dict:
     """Compulsory HBM traffic of each kernel per lattice update (values of
-    `es` bytes): stageA reads the state and writes f3 and G (78 values);
-    stageB reads f3 and G and writes the state (78). Halo re-reads, the
-    cell-type map and the wall-closure kernels are excluded."""
-    return {"stageA": 78 * es, "stageB": 78 * es}
+    `es` bytes): s12 reads the state and writes f2 (52 values); flux2 reads
+    f2 and writes the 24 net-flux planes (50); s3 reads the state and the
+    net fluxes and writes f3 and G (102); flux3 reads f3 and G and writes
+    the state (78). Halo re-reads, the cell-type map and the wall-closure
+    kernels are excluded."""
+    return {"s12": 52 * es, "flux2": 50 * es, "s3": 102 * es, "flux3": 78 * es}
 
 
 def run_ensemble(members, precision: str, steps: int, warmup: int, timing: bool):
@@ -118,7 +120,7 @@ def main() -> int:
     log(f"batched mixed: {ms:.1f} ms for {args.steps} steps -> {mlups:.1f} MLUPS (status {st})")
     kb = kernel_bytes_per_lup(4)
     ktimes = t["kernels"]
-    dom = max(("stageA", "stageB"), key=lambda k: ktimes.get(k, {}).get("avg_ms", 0.0))
+    dom = max(("s12", "flux2", "s3", "flux3"), key=lambda k: ktimes.get(k, {}).get("avg_ms", 0.0))
     dom_ms = ktimes[dom]["avg_ms"]
     achieved = kb[dom] * cells / (dom_ms * 1e-3) / 1e9
     step_bytes = 520.0  # 130 values per LUP in FP32 storage (fused two-stage model)
@@ -212,7 +214,7 @@ def main() -> int:
         r_mlups = rc.cells * args.riemann_steps / (rt["total_ms"] * 1e-3) / 1e6
         kb64 = kernel_bytes_per_lup(8)
         rk = rt["kernels"]
-        rdom = max(("stageA", "stageB"), key=lambda k: rk.get(k, {}).get("avg_ms", 0.0))
+        rdom = max(("s12", "flux2", "s3", "flux3"), key=lambda k: rk.get(k, {}).get("avg_ms", 0.0))
         r_ach = kb64[rdom] * rc.cells / (rk[rdom]["avg_ms"] * 1e-3) / 1e9
         result["riemann_fp64"] = {
             "n": args.riemann_n, "steps": args.riemann_steps, "mlups": round(r_mlups, 2),

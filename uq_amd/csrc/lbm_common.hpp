@@ -39,6 +39,7 @@ struct InstDev {
   void* state[2];      // ping-pong state buffers (26 planes each)
   void* f2;            // stage-2 distributions (26 planes)
   void* f3;            // stage-3 distributions (26 planes)
+  void* dflux;         // net fluxes dF(f2) of directions 1..12, f then h (24 planes)
   const uint8_t* ctype;
   double* alpha;       // wall closure factor, [ny][pitch]
   int* fail;           // per-instance: first failing step (INT_MAX when none)

@@ -1,26 +1,28 @@
 // lbm_kernels.hip — gfx950 kernels of the D2Q13 thermal FD-LBM.
 //
 // One step of the reference's RKMETHOD = 1 stepper (IMEX-SSP2(3,2,2),
-// reference RK3, cpuLBM.cpp:1738-2103) is two tile launches, each covering
+// reference RK3, cpuLBM.cpp:1738-2103) is four tile launches, each walking
 // every tile of every instance of the group in one grid-stride (persistent)
 // loop, plus the wall closure when the group has walls:
 //
-//   [k_s12_list  f2 of the cells the wall closure's stencils read]
-//   [k_alpha     wall cells: closure factor from the fluid neighbours' face
-//                fluxes of f2 (reference wallFluxes, cpuLBM.cpp:1502-1590)]
-//   k_stageA     stages 1-3 fused: f2 of tile + halo recomputed in LDS per
-//                direction, face fluxes, moments of f - dF(f2), stage-3
-//                equilibrium -> f3; G (stage-2 half of the final
-//                combination) -> the other state buffer
-//   [k_alpha     same for f3]
-//   k_stageB     face fluxes of f3, stage-3 relaxation -> new state
+//   k_s12      pointwise: moments, stage-1 equilibrium, implicit stages 1-2
+//              -> f2 (cpuLBM.cpp:1819-1924)
+//  [k_alpha    wall cells: closure factor from the fluid neighbours' face
+//              fluxes of f2 (wallFluxes, cpuLBM.cpp:1502-1590)]
+//   k_flux<0>  stencil: per direction, f2 of tile + 3-cell halo cross in LDS,
+//              x/y face fluxes (WENO5 or NND), net flux dF(f2) -> dF planes
+//              (posFluxes + cFLUX, cpuLBM.cpp:1436-1500)
+//   k_s3       pointwise: moments of f - dF(f2), stage-3 equilibrium,
+//              implicit stage 3 -> f3, and the final combination except its
+//              last flux term -> the other state buffer (cpuLBM.cpp:1926-2042)
+//  [k_alpha    same for f3]
+//   k_flux<1>  stencil: net flux dF(f3); new state -= dF(f3)/2
 //
-// Face fluxes are staged one direction at a time: the tile's cells plus a
-// 3-cell halo cross (rows for x fluxes, columns for y fluxes) of one f and
-// one h plane, so LDS holds 2 x 22 x 22 values for a 16 x 16 tile whatever
-// the precision. Halo positions outside the grid are resolved with the
-// reference's index() rules (periodic wrap, mirror with direction remap,
-// zeroth-order clamp; cpuLBM.cpp:1033-1169) when the tile is set up.
+// The stencil kernels run one direction at a time through a runtime loop
+// (one copy of the flux code, low register use) on 15 x 16 tiles with 256
+// threads: every thread computes exactly one x face and one y face per
+// direction (16 x 16 x-face owners, 15 x 17 y-face owners), so no wave
+// waits on a straggler at the barriers.
 #include <climits>
 #include <type_traits>
 
@@ -29,29 +31,36 @@
 
 namespace uqlbm {
 
-constexpr int kTX = 16, kTY = 16, kH = 3;
-constexpr int kNT = kTX * kTY;
+constexpr int kTX = 15, kTY = 16, kH = 3;
+constexpr int kNT = 256;
 constexpr int kRX = kTX + 2 * kH, kRY = kTY + 2 * kH;
+constexpr int kOwn = kTX * kTY;                // 240 own cells per tile
+constexpr int kNXB = 2 * kH * kTY, kNYB = 2 * kH * kTX;
+constexpr int kNDF = 24;                       // dF planes: f and h of directions 1..12
 
 // region meta bits
 enum : unsigned {
   kMetaType = 3u,
-  kMetaMirEW = 4u,   // x position reached through an east/west mirror
-  kMetaMirNS = 8u,   // y position reached through a north/south mirror
-  kMetaClampX = 16u, // x position clamped (zeroth-order extrapolation edge)
+  kMetaMirEW = 4u,    // x position reached through an east/west mirror
+  kMetaMirNS = 8u,    // y position reached through a north/south mirror
+  kMetaClampX = 16u,  // x position clamped (zeroth-order extrapolation edge)
   kMetaClampY = 32u,
-  kMetaUsed = 64u,
 };
+
+__constant__ int c_lat_x[13] = {0, 1, 0, -1, 0, 1, -1, -1, 1, 2, 0, -2, 0};
+__constant__ int c_lat_y[13] = {0, 0, 1, 0, -1, 1, 1, -1, -1, 0, 2, 0, -2};
+__constant__ int c_mir_ns[13] = {0, 1, 4, 3, 2, 8, 7, 6, 5, 9, 12, 11, 10};
+__constant__ int c_mir_ew[13] = {0, 3, 2, 1, 4, 6, 5, 8, 7, 11, 10, 9, 12};
 
 template <typename T>
 __device__ __forceinline__ T* plane(void* base, long long plane_elems, int k) {
   return reinterpret_cast<T*>(base) + plane_elems * (long long)k;
 }
 
-// Reference index() along x/y (cpuLBM.cpp:1033-1113) for one position,
-// followed by a clamp into the grid for positions no stencil reaches.
-__device__ __forceinline__ int map_axis(int X, int n, int periodic, int mir_lo, int mir_hi,
-                                        unsigned& mir, unsigned& clamped) {
+// Reference index() along one axis (cpuLBM.cpp:1033-1113) for one position,
+// then a clamp into the grid for positions no stencil reaches.
+__device__ __forceinline__ int map_axis(int X, int n, int periodic, int mir_lo, int mir_hi, unsigned& mir,
+                                        unsigned& clamped) {
   mir = 0;
   clamped = 0;
   if (X < 0) {
@@ -69,15 +78,275 @@ __device__ __forceinline__ int map_axis(int X, int n, int periodic, int mir_lo, 
 // Grid-stride tile loop with an XCD-aware order: gfx950 hands consecutive
 // workgroups to its 8 XCDs in turn, so block b is renumbered so that each
 // XCD walks a contiguous run of tiles (neighbouring tiles share halos and
-// the XCD's L2).
+// that XCD's L2).
 __device__ __forceinline__ int first_tile() {
   const int G = gridDim.x, b = blockIdx.x;
   if (G % 8 != 0) return b;
   return (b % 8) * (G / 8) + b / 8;
 }
 
+template <int... Is>
+struct Seq {};
+template <int N, int... Is>
+struct MakeSeq : MakeSeq<N - 1, N - 1, Is...> {};
+template <int... Is>
+struct MakeSeq<0, Is...> {
+  using type = Seq<Is...>;
+};
+
+// `step` counts the steps of the current launch sequence from 1; the global
+// step index of an instance is in.step_base + step. A failure flagged during
+// step s freezes the instance for steps > s only, so this test gives the
+// same answer to every thread of a launch.
+__device__ __forceinline__ bool frozen_by_failure(const InstDev& in, int step) {
+  return *in.fail < in.step_base + step;
+}
+
+__device__ __forceinline__ void flag_failure(const InstDev& in, int step, double rho, double sumfeq, double tol) {
+  // reference check |sum feq - rho| > 1e-6 (cpuLBM.cpp:825-843); a
+  // non-finite density also fails (the reference misses NaN).
+  if (!(fabs(sumfeq - rho) <= tol) || !isfinite(rho)) atomicMin(in.fail, in.step_base + step);
+}
+
+// Tolerance of the equilibrium mass check: the reference's 1e-6 for FP64
+// arithmetic; for FP32 equilibria the rounding bound of the FP32 sum is added
+// so that rounding alone never trips it.
+template <typename TC>
+__device__ __forceinline__ double eq_check_tol(double sum_abs) {
+  return std::is_same<TC, double>::value ? 1e-6 : 1e-6 + 16.0 * 5.96e-8 * sum_abs;
+}
+
 // ---------------------------------------------------------------------------
-// Tile region: physical offsets and meta of the tile plus halo cross.
+// Pointwise helpers
+// ---------------------------------------------------------------------------
+template <typename TS>
+__device__ __forceinline__ void load26(const TS* src, long long P, long long o, double* f, double* h) {
+#pragma unroll
+  for (int i = 0; i < kQ; i++) {
+    f[i] = double(src[P * i + o]);
+    h[i] = double(src[P * (kQ + i) + o]);
+  }
+}
+
+__device__ __forceinline__ void moments26(const InstDev& in, const double* f, const double* h, Macro& m) {
+  double rho = 0, rux = 0, ruy = 0, sh = 0;
+#pragma unroll
+  for (int i = 0; i < kQ; i++) {
+    rho += f[i];
+    rux += f[i] * (lat_x(i) * in.c);
+    ruy += f[i] * (lat_y(i) * in.c);
+    sh += h[i];
+  }
+  finish_macro(in, rho, rux, ruy, sh, m);
+}
+
+// Stage-1/2 coefficients of one cell.
+template <typename TC>
+struct Rec12 {
+  EqIn<TC> q;
+  TC cU, cV;          // c*U, c*V: e_i . u = lat_x(i)*cU + lat_y(i)*cV
+  TC af, ah, ahf, g;  // dt/(2 tf), dt/(2 th), dt/(2 thf), 1/thf
+};
+
+template <typename TC>
+__device__ __forceinline__ Rec12<TC> make_rec12(const InstDev& in, const Macro& m) {
+  Rec12<TC> r;
+  r.q = eq_inputs<TC>(in, m);
+  r.cU = TC(in.c) * TC(m.ux);
+  r.cV = TC(in.c) * TC(m.uy);
+  const TC dt = TC(in.dt);
+  r.af = dt / (TC(2) * TC(m.tf));
+  r.ah = dt / (TC(2) * TC(m.th));
+  r.ahf = dt / (TC(2) * TC(m.thf));
+  r.g = TC(1) / TC(m.thf);
+  return r;
+}
+
+// f2, h2 of direction I (reference stages 1-2, cpuLBM.cpp:1853, 1862, 1905,
+// 1920) and the stage-1 equilibria fe, he (feq2 = feq1).
+template <int I, typename TC>
+__device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC& fe, TC& he, TC& f2, TC& h2) {
+  eq_dir<I>(r.q, fe, he);
+  const TC f1 = (fv + r.af * fe) / (TC(1) + r.af);
+  const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
+  const TC h1 = (hv - r.ahf * eu * (fe - f1) + r.ah * he) / (TC(1) + r.ah);
+  f2 = (fv - r.af * (fe - f1) + r.af * fe) / (TC(1) + r.af);
+  h2 = (hv - hdt * ((eu * r.g) * (fe - f2) - (eu * r.g) * (fe - f1)) - r.ah * (he - h1) + r.ah * he) /
+       (TC(1) + r.ah);
+}
+
+// Own cell of a pointwise tile launch (threads with lx == 15 have none).
+struct PointCell {
+  long long o;
+  bool active;
+};
+
+__device__ __forceinline__ PointCell point_cell(const InstDev& in, const TileRef& tr) {
+  const int lx = (int)threadIdx.x % 16, ly = (int)threadIdx.x / 16;
+  const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
+  PointCell c;
+  c.active = lx < kTX && x < in.nx && y < in.ny;
+  c.o = c.active ? (long long)y * in.pitch + x : 0;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// k_s12: RK3 stages 1 and 2, pointwise -> f2.
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int... Is>
+__device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, TS* dst, long long o, int step,
+                                         Seq<Is...>) {
+  const long long P = in.plane;
+  double f[kQ], h[kQ];
+  load26(src, P, o, f, h);
+  Macro m;
+  moments26(in, f, h, m);
+  const Rec12<TC> r = make_rec12<TC>(in, m);
+  const TC hdt = TC(in.dt) / TC(2);
+  double sumfe = 0, sumabs = 0;
+  auto one = [&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    TC fe, he, f2, h2;
+    s12<I>(r, hdt, TC(f[I]), TC(h[I]), fe, he, f2, h2);
+    sumfe += double(fe);
+    sumabs += fabs(double(fe));
+    dst[P * I + o] = TS(f2);
+    dst[P * (kQ + I) + o] = TS(h2);
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+  flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
+}
+
+template <typename TS, typename TC>
+__global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                             int step) {
+  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const InstDev& in = insts[tr.inst];
+    if (frozen_by_failure(in, step)) continue;
+    const PointCell c = point_cell(in, tr);
+    if (!c.active) continue;
+    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+    TS* dst = reinterpret_cast<TS*>(in.f2);
+    if (in.ctype[c.o] > 0) {
+#pragma unroll
+      for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
+      continue;
+    }
+    s12_cell<TS, TC>(in, src, dst, c.o, step, MakeSeq<kQ>::type{});
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_s3: RK3 stage 3 and the final combination up to its dF(f3) term
+// (cpuLBM.cpp:1926-1993, 2009-2040), pointwise, one pass over the
+// directions. With P = f - dF2 +
+// af2 (feq2 - f2) and Q = (h - dH2) - dt/2 A2 + ah2 (heq2 - h2):
+//   f3 = (P + af3 feq3)/(1 + af3),  h3 = (Q - dt/2 A3 + ah3 heq3)/(1 + ah3)
+//   G  = P + dF2/2 + dt/2 (feq3 - f3)/tf3                       (f part)
+//   G  = Q + dH2/2 + dt/2 (heq3 - h3)/th3 - dt/2 A3              (h part)
+// and the new state is G - dF(f3)/2; A = (e.u/thf)(feq - f). (P + dF2/2 is
+// f - dF2/2 + dt/2 (feq2 - f2)/tf2 since af2 = dt/(2 tf2).)
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int... Is>
+__device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const TS* dfb, TS* f3b, TS* out,
+                                        long long o, int step, Seq<Is...>) {
+  const long long P = in.plane;
+  double f[kQ], h[kQ];
+  load26(src, P, o, f, h);
+  // stage-1 moments, and the stage-3 moments: those of f - dF(f2)
+  // (macroProp3, cpuLBM.cpp:973-1032), summed as moments of f minus moments
+  // of dF so that every direction can be finished in one pass
+  double rho = 0, rux = 0, ruy = 0, sh = 0, drho = 0, drux = 0, druy = 0, dsh = 0;
+#pragma unroll
+  for (int i = 0; i < kQ; i++) {
+    const double exc = lat_x(i) * in.c, eyc = lat_y(i) * in.c;
+    rho += f[i];
+    rux += f[i] * exc;
+    ruy += f[i] * eyc;
+    sh += h[i];
+    if (i) {
+      const double dF = double(dfb[P * (i - 1) + o]), dH = double(dfb[P * (12 + i - 1) + o]);
+      drho += dF;
+      drux += dF * exc;
+      druy += dF * eyc;
+      dsh += dH;
+    }
+  }
+  Macro m1, m3;
+  finish_macro(in, rho, rux, ruy, sh, m1);
+  finish_macro(in, rho - drho, rux - drux, ruy - druy, sh - dsh, m3);
+  const Rec12<TC> r = make_rec12<TC>(in, m1);
+  const TC dt = TC(in.dt), hdt = dt / TC(2);
+  const TC tf2 = TC(m1.tf), th2 = TC(m1.th), thf2 = TC(m1.thf);
+  const TC af2 = dt / (TC(2) * tf2), ah2 = dt / (TC(2) * th2);
+  const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
+  const TC tf3 = TC(m3.tf), th3 = TC(m3.th), thf3 = TC(m3.thf);
+  const TC af3 = dt / (TC(2) * tf3), ah3 = dt / (TC(2) * th3);
+  const TC cU3 = TC(in.c) * TC(m3.ux), cV3 = TC(in.c) * TC(m3.uy);
+  double sumfe = 0, sumabs = 0;
+  auto one = [&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    TC dF = 0, dH = 0;
+    if constexpr (I != 0) {
+      dF = TC(dfb[P * (I - 1) + o]);
+      dH = TC(dfb[P * (12 + I - 1) + o]);
+    }
+    const TC fv = TC(f[I]), hv = TC(h[I]);
+    TC fe, he, f2, h2;
+    s12<I>(r, hdt, fv, hv, fe, he, f2, h2);
+    const TC D = fe - f2;
+    const TC eu2 = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
+    const TC A2 = (eu2 / thf2) * D;
+    const TC Pf = (fv - dF) + af2 * D;
+    const TC Qh = ((hv - dH) - hdt * A2) + ah2 * (he - h2);
+    TC fe3, he3;
+    eq_dir<I>(q3, fe3, he3);
+    sumfe += double(fe3);
+    sumabs += fabs(double(fe3));
+    const TC f3 = (Pf + af3 * fe3) / (TC(1) + af3);
+    const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
+    const TC A3 = (eu3 / thf3) * (fe3 - f3);
+    const TC h3 = ((Qh - hdt * A3) + ah3 * he3) / (TC(1) + ah3);
+    const TC gf = (Pf + TC(0.5) * dF) + hdt * ((fe3 - f3) / tf3);
+    const TC gh = ((Qh + TC(0.5) * dH) + hdt * ((he3 - h3) / th3)) - hdt * A3;
+    f3b[P * I + o] = TS(f3);
+    f3b[P * (kQ + I) + o] = TS(h3);
+    out[P * I + o] = TS(gf);
+    out[P * (kQ + I) + o] = TS(gh);
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+  flag_failure(in, step, m3.rho, sumfe, eq_check_tol<TC>(sumabs));
+}
+
+template <typename TS, typename TC>
+__global__ __launch_bounds__(kNT) void k_s3(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                            int step) {
+  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const InstDev& in = insts[tr.inst];
+    if (frozen_by_failure(in, step)) continue;
+    const PointCell c = point_cell(in, tr);
+    if (!c.active) continue;
+    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+    TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+    TS* f3b = reinterpret_cast<TS*>(in.f3);
+    if (in.ctype[c.o] > 0) {
+      // frozen cell: f3 = f2 = f, and the new state is f
+#pragma unroll
+      for (int k = 0; k < kPlanes; k++) {
+        const TS v = src[in.plane * k + c.o];
+        f3b[in.plane * k + c.o] = v;
+        out[in.plane * k + c.o] = v;
+      }
+      continue;
+    }
+    s3_cell<TS, TC>(in, src, reinterpret_cast<const TS*>(in.dflux), f3b, out, c.o, step, MakeSeq<kQ>::type{});
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stencil tiles: region (tile + 3-cell halo cross), staging and faces.
 // ---------------------------------------------------------------------------
 struct Region {
   int off[kRY][kRX];
@@ -106,68 +375,32 @@ __device__ void setup_region(Region& R, const InstDev& in, int x0, int y0) {
     m |= my ? kMetaMirNS : 0u;
     m |= cx ? kMetaClampX : 0u;
     m |= cy ? kMetaClampY : 0u;
-    R.meta[ly + kH][lx + kH] = (unsigned char)(m | kMetaUsed);
+    R.meta[ly + kH][lx + kH] = (unsigned char)m;
   }
 }
 
-__device__ __forceinline__ int remap_dir(int i, unsigned m) {
-  if (m & kMetaMirEW) i = mirror_ew(i);
-  if (m & kMetaMirNS) i = mirror_ns(i);
-  return i;
-}
-
-// Staging buffers and face fluxes of one direction.
 template <typename TC>
-struct Stage {
+struct FluxStage {
   TC f[kRY][kRX];
   TC h[kRY][kRX];
-  TC fxf[kTY][kTX + 1], fxh[kTY][kTX + 1];
-  TC fyf[kTY + 1][kTX], fyh[kTY + 1][kTX];
+  TC fxf[kTY][kTX + 1], fxh[kTY][kTX + 1];   // x faces: owner column lx - lo
+  TC fyf[kTY + 1][kTX], fyh[kTY + 1][kTX];   // y faces: owner row ly - lo
 };
 
-// Copy plane I (mirror-remapped per position) of `src` into the staging
-// buffers for the own rect and the halo bands direction I needs.
-template <int I, typename TS, typename TC>
-__device__ __forceinline__ void stage_raw(Stage<TC>& S, const Region& R, const InstDev& in, void* src) {
-  constexpr int EX = lat_x(I), EY = lat_y(I);
-  constexpr int nown = kNT, nxb = EX ? 2 * kH * kTY : 0, nyb = EY ? 2 * kH * kTX : 0;
-  for (int r = threadIdx.x; r < nown + nxb + nyb; r += kNT) {
-    int lx, ly;
-    if (r < nown) {
-      lx = r % kTX;
-      ly = r / kTX;
-    } else if (r < nown + nxb) {
-      const int k = r - nown, side = k / (kH * kTY), kk = k % (kH * kTY);
-      ly = kk / kH;
-      lx = side ? kTX + kk % kH : -kH + kk % kH;
-    } else {
-      const int k = r - nown - nxb, side = k / (kH * kTX), kk = k % (kH * kTX);
-      lx = kk % kTX;
-      ly = side ? kTY + kk / kTX : -kH + kk / kTX;
-    }
-    const unsigned m = R.meta[ly + kH][lx + kH];
-    const int d = remap_dir(I, m);
-    const long long o = R.off[ly + kH][lx + kH];
-    S.f[ly + kH][lx + kH] = TC(plane<TS>(src, in.plane, d)[o]);
-    S.h[ly + kH][lx + kH] = TC(plane<TS>(src, in.plane, kQ + d)[o]);
-  }
-}
-
 // Reference stencil solid handling along one line (cpuLBM.cpp:1242-1322):
-// the first solid/fluid transition's solid point becomes 2*V(m1) - V(m2) of
-// the two nearest fluid points, and the point beyond it the same value.
-// The addressing equivalence with the reference (positions inside the
-// stencil, unmirrored direction) is checked on the host per geometry.
+// the first solid/fluid transition's solid point becomes 2 V(m1) - V(m2) of
+// the two nearest fluid points, and the point beyond it the same value. The
+// addressing equivalence with the reference is checked on the host per
+// geometry (lbm_runtime.hip, validate_geometry).
 template <int L, typename TC>
 __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) {
   int diff = 0, jj = 0;
 #pragma unroll
   for (int j = 0; j < L - 1; j++) {
     const int d = sld[j] - sld[j + 1];
-    if (d != 0) {
+    if (d != 0 && diff == 0) {
       diff = d;
       jj = d > 0 ? j : j + 1;
-      break;
     }
   }
   int k1 = jj + diff, k2 = jj + 2 * diff;
@@ -178,15 +411,12 @@ __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) 
   if (nx >= 0 && nx < L) S[nx] = S[jj];
 }
 
-// Face flux (f and h) out of the owner at logical (lx, ly) along axis AX for
-// direction I: the wall closure at solid owners, else WENO5/NND on the
-// staged line.
-template <int I, int AX, typename TC>
-__device__ __forceinline__ void face_flux(const Stage<TC>& S, const Region& R, const InstDev& in,
-                                          bool solids, int lx, int ly, TC& vf, TC& vh) {
-  constexpr int E = AX == 0 ? lat_x(I) : lat_y(I);
-  constexpr int SG = sgn_i(E);
-  const TC v = TC(double(E > 0 ? E : -E) * in.c);
+// Face flux (f and h) out of the owner at logical (lx, ly) along axis AX with
+// stencil step sg and speed v = |e_axis|: the wall closure at solid owners,
+// else WENO5/NND on the staged line.
+template <int AX, typename TC>
+__device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& R, const InstDev& in, bool solids,
+                                          int lx, int ly, int sg, TC v, TC& vf, TC& vh) {
   const unsigned mo = R.meta[ly + kH][lx + kH];
   if ((mo & kMetaType) == 2) {
     const TC a = TC(in.alpha[R.off[ly + kH][lx + kH]]);
@@ -194,450 +424,178 @@ __device__ __forceinline__ void face_flux(const Stage<TC>& S, const Region& R, c
     vh = a * v * S.h[ly + kH][lx + kH];
     return;
   }
+  const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
   if (in.flux_method == 1) {
     TC Vf[5], Vh[5];
+    int sld[5], any = 0;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      const int px = AX == 0 ? lx + (k - 2) * SG : lx;
-      const int py = AX == 0 ? ly : ly + (k - 2) * SG;
-      Vf[k] = S.f[py + kH][px + kH];
-      Vh[k] = S.h[py + kH][px + kH];
+      const int px = lx + (k - 2) * dx + kH, py = ly + (k - 2) * dy + kH;
+      Vf[k] = S.f[py][px];
+      Vh[k] = S.h[py][px];
+      sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
+      any |= sld[k];
     }
-    if (solids) {
-      int sld[5], any = 0;
+    if (any) {
+      TC Sf[5], Sh[5];
 #pragma unroll
-      for (int k = 0; k < 5; k++) {
-        const int px = AX == 0 ? lx + (k - 2) * SG : lx;
-        const int py = AX == 0 ? ly : ly + (k - 2) * SG;
-        sld[k] = (R.meta[py + kH][px + kH] & kMetaType) == 2;
-        any |= sld[k];
-      }
-      if (any) {
-        TC Sf[5], Sh[5];
+      for (int k = 0; k < 5; k++) { Sf[k] = Vf[k]; Sh[k] = Vh[k]; }
+      extrapolate<5>(sld, Vf, Sf);
+      extrapolate<5>(sld, Vh, Sh);
 #pragma unroll
-        for (int k = 0; k < 5; k++) { Sf[k] = Vf[k]; Sh[k] = Vh[k]; }
-        extrapolate<5>(sld, Vf, Sf);
-        extrapolate<5>(sld, Vh, Sh);
-        vf = weno5(v, Sf[0], Sf[1], Sf[2], Sf[3], Sf[4]);
-        vh = weno5(v, Sh[0], Sh[1], Sh[2], Sh[3], Sh[4]);
-        return;
-      }
+      for (int k = 0; k < 5; k++) { Vf[k] = Sf[k]; Vh[k] = Sh[k]; }
     }
     vf = weno5(v, Vf[0], Vf[1], Vf[2], Vf[3], Vf[4]);
     vh = weno5(v, Vh[0], Vh[1], Vh[2], Vh[3], Vh[4]);
   } else {
     TC Vf[3], Vh[3];
+    int sld[3], any = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const int px = AX == 0 ? lx + (k - 1) * SG : lx;
-      const int py = AX == 0 ? ly : ly + (k - 1) * SG;
-      Vf[k] = S.f[py + kH][px + kH];
-      Vh[k] = S.h[py + kH][px + kH];
+      const int px = lx + (k - 1) * dx + kH, py = ly + (k - 1) * dy + kH;
+      Vf[k] = S.f[py][px];
+      Vh[k] = S.h[py][px];
+      sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
+      any |= sld[k];
     }
-    if (solids) {
-      int sld[3], any = 0;
+    if (any) {
+      TC Sf[3], Sh[3];
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        const int px = AX == 0 ? lx + (k - 1) * SG : lx;
-        const int py = AX == 0 ? ly : ly + (k - 1) * SG;
-        sld[k] = (R.meta[py + kH][px + kH] & kMetaType) == 2;
-        any |= sld[k];
-      }
-      if (any) {
-        TC Sf[3], Sh[3];
+      for (int k = 0; k < 3; k++) { Sf[k] = Vf[k]; Sh[k] = Vh[k]; }
+      extrapolate<3>(sld, Vf, Sf);
+      extrapolate<3>(sld, Vh, Sh);
 #pragma unroll
-        for (int k = 0; k < 3; k++) { Sf[k] = Vf[k]; Sh[k] = Vh[k]; }
-        extrapolate<3>(sld, Vf, Sf);
-        extrapolate<3>(sld, Vh, Sh);
-        vf = nnd(v, Sf[0], Sf[1], Sf[2]);
-        vh = nnd(v, Sh[0], Sh[1], Sh[2]);
-        return;
-      }
+      for (int k = 0; k < 3; k++) { Vf[k] = Sf[k]; Vh[k] = Sh[k]; }
     }
     vf = nnd(v, Vf[0], Vf[1], Vf[2]);
     vh = nnd(v, Vh[0], Vh[1], Vh[2]);
   }
 }
 
-// All face fluxes direction I needs for the tile's cells: each cell's own
-// out-face plus one upwind owner per row (x) / column (y).
-template <int I, typename TC>
-__device__ __forceinline__ void tile_faces(Stage<TC>& S, const Region& R, const InstDev& in, bool solids) {
-  constexpr int EX = lat_x(I), EY = lat_y(I);
-  constexpr int SX = sgn_i(EX), SY = sgn_i(EY);
-  constexpr int nfx = EX ? (kTX + 1) * kTY : 0;
-  constexpr int nfy = EY ? (kTY + 1) * kTX : 0;
-  for (int k = threadIdx.x; k < nfx + nfy; k += kNT) {
-    if (k < nfx) {
-      const int row = k / (kTX + 1), col = k % (kTX + 1);
-      const int lo = SX > 0 ? -1 : 0;
+// ---------------------------------------------------------------------------
+// k_flux<MODE>: net flux of every direction for the tile's cells.
+//   MODE 0: source f2; dF, dH -> the dF planes.
+//   MODE 1: source f3; new state (holding G) -= dF/2, dH/2.
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int MODE>
+__device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
+                                          int par) {
+  const int t = threadIdx.x;
+  const int tx = t % 16, ty = t / 16;
+  const int x = tr.tx * kTX + tx, y = tr.ty * kTY + ty;
+  const bool own = tx < kTX && x < in.nx && y < in.ny;
+  const bool fluid = own && (R.meta[ty + kH][tx + kH] & kMetaType) == 0;
+  const long long oo = own ? R.off[ty + kH][tx + kH] : 0;
+  const bool solids = tr.flags & kTileSolid;
+  const long long P = in.plane;
+  void* src = MODE == 0 ? in.f2 : in.f3;
+  TS* dst = MODE == 0 ? reinterpret_cast<TS*>(in.dflux) : reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+  const TC rdx = TC(in.rdx), rdy = TC(in.rdy);
+  for (int I = 1; I < kQ; I++) {
+    const int ex = c_lat_x[I], ey = c_lat_y[I];
+    const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
+    const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
+    // staging: own rect, then the halo bands this direction needs
+    const int nxb = ex ? kNXB : 0, nyb = ey ? kNYB : 0;
+    for (int r = t; r < kOwn + nxb + nyb; r += kNT) {
+      int lx, ly;
+      if (r < kOwn) {
+        lx = r % kTX;
+        ly = r / kTX;
+      } else if (r < kOwn + nxb) {
+        const int k = r - kOwn, side = k / (kH * kTY), kk = k % (kH * kTY);
+        ly = kk / kH;
+        lx = side ? kTX + kk % kH : -kH + kk % kH;
+      } else {
+        const int k = r - kOwn - nxb, side = k / (kH * kTX), kk = k % (kH * kTX);
+        lx = kk % kTX;
+        ly = side ? kTY + kk / kTX : -kH + kk / kTX;
+      }
+      const unsigned m = R.meta[ly + kH][lx + kH];
+      const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
+      const long long o = R.off[ly + kH][lx + kH];
+      S.f[ly + kH][lx + kH] = TC(plane<TS>(src, P, d)[o]);
+      S.h[ly + kH][lx + kH] = TC(plane<TS>(src, P, kQ + d)[o]);
+    }
+    __syncthreads();
+    // faces: thread t owns x face (row ty, owner column tx - [sx > 0]) and
+    // y face (column t % 15, owner row t / 15 - [sy > 0])
+    if (ex) {
+      const TC v = TC(double(ex > 0 ? ex : -ex) * in.c);
       TC vf, vh;
-      face_flux<I, 0>(S, R, in, solids, lo + col, row, vf, vh);
-      S.fxf[row][col] = vf;
-      S.fxh[row][col] = vh;
-    } else {
-      const int kk = k - nfx;
-      const int row = kk / kTX, col = kk % kTX;
-      const int lo = SY > 0 ? -1 : 0;
+      face_flux<0, TC>(S, R, in, solids, tx - (sx > 0 ? 1 : 0), ty, sx, v, vf, vh);
+      S.fxf[ty][tx] = vf;
+      S.fxh[ty][tx] = vh;
+    }
+    if (ey && t < kTX * (kTY + 1)) {
+      const int col = t % kTX, row = t / kTX;
+      const TC v = TC(double(ey > 0 ? ey : -ey) * in.c);
       TC vf, vh;
-      face_flux<I, 1>(S, R, in, solids, col, lo + row, vf, vh);
+      face_flux<1, TC>(S, R, in, solids, col, row - (sy > 0 ? 1 : 0), sy, v, vf, vh);
       S.fyf[row][col] = vf;
       S.fyh[row][col] = vh;
     }
-  }
-}
-
-// Net flux of direction I at the own cell (reference cFLUX,
-// cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) + (dt/dy)(Fy_out - Fy_in),
-// grouped as out - in. An upwind owner across a clamped edge is the cell
-// itself.
-template <int I, typename TC>
-__device__ __forceinline__ void own_net_flux(const Stage<TC>& S, const Region& R, const InstDev& in,
-                                             int lx, int ly, TC& dF, TC& dH) {
-  constexpr int EX = lat_x(I), EY = lat_y(I);
-  constexpr int SX = sgn_i(EX), SY = sgn_i(EY);
-  const TC rdx = TC(in.rdx), rdy = TC(in.rdy);
-  TC oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
-  if constexpr (EX != 0) {
-    const int lo = SX > 0 ? -1 : 0;
-    oxf = S.fxf[ly][lx - lo];
-    oxh = S.fxh[ly][lx - lo];
-    const int ux = lx - SX;
-    if (R.meta[ly + kH][ux + kH] & kMetaClampX) {
-      ixf = oxf;
-      ixh = oxh;
-    } else {
-      ixf = S.fxf[ly][ux - lo];
-      ixh = S.fxh[ly][ux - lo];
-    }
-  }
-  if constexpr (EY != 0) {
-    const int lo = SY > 0 ? -1 : 0;
-    oyf = S.fyf[ly - lo][lx];
-    oyh = S.fyh[ly - lo][lx];
-    const int uy = ly - SY;
-    if (R.meta[uy + kH][lx + kH] & kMetaClampY) {
-      iyf = oyf;
-      iyh = oyh;
-    } else {
-      iyf = S.fyf[uy - lo][lx];
-      iyh = S.fyh[uy - lo][lx];
-    }
-  }
-  if constexpr (EX != 0 && EY != 0) {
-    dF = (rdx * oxf + rdy * oyf) - (rdx * ixf + rdy * iyf);
-    dH = (rdx * oxh + rdy * oyh) - (rdx * ixh + rdy * iyh);
-  } else if constexpr (EX != 0) {
-    dF = rdx * oxf - rdx * ixf;
-    dH = rdx * oxh - rdx * ixh;
-  } else {
-    dF = rdy * oyf - rdy * iyf;
-    dH = rdy * oyh - rdy * iyh;
-  }
-}
-
-template <int... Is>
-struct Seq {};
-template <int N, int... Is>
-struct MakeSeq : MakeSeq<N - 1, N - 1, Is...> {};
-template <int... Is>
-struct MakeSeq<0, Is...> {
-  using type = Seq<Is...>;
-};
-
-// `step` counts the steps of the current launch sequence from 1; the global
-// step index of an instance is in.step_base + step.
-__device__ __forceinline__ bool frozen_by_failure(const InstDev& in, int step) {
-  return *in.fail < in.step_base + step;
-}
-
-__device__ __forceinline__ void flag_failure(const InstDev& in, int step, double rho, double sumfeq) {
-  if (!(fabs(sumfeq - rho) <= 1e-6) || !isfinite(rho)) {
-    // reference check: |sum feq - rho| > 1e-6 (cpuLBM.cpp:825-843); a
-    // non-finite density also fails here (the reference misses NaN).
-    atomicMin(in.fail, in.step_base + step);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// RK3 stages 1 and 2 (cpuLBM.cpp:1819-1924) are pointwise. Their per-cell
-// coefficients live in a record; positions reached through a mirror store the
-// mirrored velocity, since eq_I(u -> -u) == eq_{mirror_ew(I)}(u) and likewise
-// for v, so one direction I serves every halo position.
-// ---------------------------------------------------------------------------
-template <typename TC>
-struct Rec12 {
-  EqIn<TC> q;
-  TC cU, cV;          // c*U, c*V: e_i . u = lat_x(i)*cU + lat_y(i)*cV
-  TC af, ah, ahf, g;  // dt/(2 tf), dt/(2 th), dt/(2 thf), 1/thf
-};
-
-template <typename TC>
-__device__ __forceinline__ Rec12<TC> make_rec12(const InstDev& in, Macro m, bool flip_x, bool flip_y) {
-  if (flip_x) m.ux = -m.ux;
-  if (flip_y) m.uy = -m.uy;
-  Rec12<TC> r;
-  r.q = eq_inputs<TC>(in, m);
-  r.cU = TC(in.c) * TC(m.ux);
-  r.cV = TC(in.c) * TC(m.uy);
-  const TC dt = TC(in.dt);
-  r.af = dt / (TC(2) * TC(m.tf));
-  r.ah = dt / (TC(2) * TC(m.th));
-  r.ahf = dt / (TC(2) * TC(m.thf));
-  r.g = TC(1) / TC(m.thf);
-  return r;
-}
-
-// f2, h2 of direction I from the raw f, h and the cell record; also returns
-// the stage-1 equilibria (feq2 = feq1, reference cpuLBM.cpp:1874-1875).
-template <int I, typename TC>
-__device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC& fe, TC& he, TC& f2, TC& h2) {
-  eq_dir<I>(r.q, fe, he);
-  const TC f1 = (fv + r.af * fe) / (TC(1) + r.af);
-  const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
-  const TC h1 = (hv - r.ahf * eu * (fe - f1) + r.ah * he) / (TC(1) + r.ah);
-  f2 = (fv - r.af * (fe - f1) + r.af * fe) / (TC(1) + r.af);
-  h2 = (hv - hdt * ((eu * r.g) * (fe - f2) - (eu * r.g) * (fe - f1)) - r.ah * (he - h1) + r.ah * he) /
-       (TC(1) + r.ah);
-}
-
-template <typename TS>
-__device__ __forceinline__ void load26(const TS* src, long long P, long long o, double* f, double* h) {
-#pragma unroll
-  for (int i = 0; i < kQ; i++) {
-    f[i] = double(src[P * i + o]);
-    h[i] = double(src[P * (kQ + i) + o]);
-  }
-}
-
-__device__ __forceinline__ void moments26(const InstDev& in, const double* f, const double* h, Macro& m) {
-  double rho = 0, rux = 0, ruy = 0, sh = 0;
-#pragma unroll
-  for (int i = 0; i < kQ; i++) {
-    rho += f[i];
-    rux += f[i] * (lat_x(i) * in.c);
-    ruy += f[i] * (lat_y(i) * in.c);
-    sh += h[i];
-  }
-  finish_macro(in, rho, rux, ruy, sh, m);
-}
-
-// Halo positions of a tile: the x band (H columns either side, tile rows)
-// then the y band (H rows either side, tile columns).
-constexpr int kNXB = 2 * kH * kTY, kNYB = 2 * kH * kTX, kNHalo = kNXB + kNYB;
-
-__device__ __forceinline__ void halo_pos(int k, int& lx, int& ly) {
-  if (k < kNXB) {
-    const int side = k / (kH * kTY), kk = k % (kH * kTY);
-    ly = kk / kH;
-    lx = side ? kTX + kk % kH : -kH + kk % kH;
-  } else {
-    const int k2 = k - kNXB, side = k2 / (kH * kTX), kk = k2 % (kH * kTX);
-    lx = kk % kTX;
-    ly = side ? kTY + kk / kTX : -kH + kk / kTX;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_s12_list: f2 for a list of cells (the stencil cells of the wall closure).
-// ---------------------------------------------------------------------------
-template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, TS* dst, long long o, Seq<Is...>) {
-  const long long P = in.plane;
-  double f[kQ], h[kQ];
-  load26(src, P, o, f, h);
-  Macro m;
-  moments26(in, f, h, m);
-  const Rec12<TC> r = make_rec12<TC>(in, m, false, false);
-  const TC hdt = TC(in.dt) / TC(2);
-  auto one = [&](auto Ic) {
-    constexpr int I = decltype(Ic)::value;
-    TC fe, he, f2, h2;
-    s12<I>(r, hdt, TC(f[I]), TC(h[I]), fe, he, f2, h2);
-    dst[P * I + o] = TS(f2);
-    dst[P * (kQ + I) + o] = TS(h2);
-  };
-  (one(std::integral_constant<int, Is>{}), ...);
-}
-
-template <typename TS, typename TC>
-__global__ __launch_bounds__(256) void k_s12_list(const InstDev* insts, const WallRef* cells, int ncells, int par,
-                                                   int step) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ncells) return;
-  const WallRef cr = cells[k];
-  const InstDev& in = insts[cr.inst];
-  if (frozen_by_failure(in, step)) return;
-  const long long o = (long long)cr.y * in.pitch + cr.x;
-  const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
-  TS* dst = reinterpret_cast<TS*>(in.f2);
-  if (in.ctype[o] > 0) {
-#pragma unroll
-    for (int q = 0; q < kPlanes; q++) dst[in.plane * q + o] = src[in.plane * q + o];
-    return;
-  }
-  s12_cell<TS, TC>(in, src, dst, o, MakeSeq<kQ>::type{});
-}
-
-// ---------------------------------------------------------------------------
-// k_stageA: RK3 stages 1-3 fused (cpuLBM.cpp:1819-1993) plus the stage-2
-// half of the final combination (:2022, :2039).
-//
-// Per tile: stage-1/2 records for the halo cross go to LDS; per direction
-// the f2/h2 plane of tile + halo is computed into the staging buffer, face
-// fluxes follow, and each cell keeps two values (P = f - dF2 + af2 (feq2 -
-// f2), Q likewise for h) while G = f - dF2/2 + dt/2 (feq2 - f2)/tf2 (and the
-// h analogue) goes straight to the output state buffer. After the last
-// direction the moments of f - dF(f2) give the stage-3 equilibrium and f3.
-// ---------------------------------------------------------------------------
-template <typename TC>
-struct HaloStore {
-  Rec12<TC> rec[kNHalo];
-};
-
-template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void stageA_tile(Stage<TC>& S, HaloStore<TC>& HS, const Region& R, const InstDev& in,
-                                            const TileRef& tr, int par, int step, Seq<Is...>) {
-  const int lx = (int)threadIdx.x % kTX, ly = (int)threadIdx.x / kTX;
-  const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
-  const bool own_in = x < in.nx && y < in.ny;
-  const unsigned mo = R.meta[ly + kH][lx + kH];
-  const long long oo = R.off[ly + kH][lx + kH];   // mapped physical cell of the own position
-  const int type_o = mo & kMetaType;
-  const bool fluid = own_in && type_o == 0;
-  const bool solids = tr.flags & kTileSolid;
-  const long long P = in.plane;
-  const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
-  TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
-  TS* f3buf = reinterpret_cast<TS*>(in.f3);
-  const TC dt = TC(in.dt), hdt = dt / TC(2);
-
-  // --- phase 1: own record (registers) and halo records (LDS) -------------
-  double fr[kQ], hr[kQ];
-  load26(src, P, oo, fr, hr);
-  Macro m1;
-  moments26(in, fr, hr, m1);
-  const Rec12<TC> ro = make_rec12<TC>(in, m1, mo & kMetaMirEW, mo & kMetaMirNS);
-  for (int k = threadIdx.x; k < kNHalo; k += kNT) {
-    int hx, hy;
-    halo_pos(k, hx, hy);
-    const unsigned mh = R.meta[hy + kH][hx + kH];
-    if ((mh & kMetaType) != 0) continue;
-    double f[kQ], h[kQ];
-    load26(src, P, R.off[hy + kH][hx + kH], f, h);
-    Macro m;
-    moments26(in, f, h, m);
-    HS.rec[k] = make_rec12<TC>(in, m, mh & kMetaMirEW, mh & kMetaMirNS);
-  }
-  __syncthreads();
-
-  TC Pf[kQ], Qh[kQ];
-  double rho3 = 0, rux3 = 0, ruy3 = 0, sh3 = 0;
-  const TC tf2 = TC(m1.tf), th2 = TC(m1.th), thf2 = TC(m1.thf);
-  const TC af2 = dt / (TC(2) * tf2), ah2 = dt / (TC(2) * th2);
-
-  auto one_dir = [&](auto Ic) {
-    constexpr int I = decltype(Ic)::value;
-    constexpr int EX = lat_x(I), EY = lat_y(I);
-    // own position: f2/h2 from the record (raw value of the mirrored direction
-    // when the position lies outside the grid)
-    TC fe = 0, he = 0, f2 = 0, h2 = 0;
-    {
-      const int d = remap_dir(I, mo);
-      const TC fv = own_in ? TC(fr[I]) : TC(plane<TS>((void*)src, P, d)[oo]);
-      const TC hv = own_in ? TC(hr[I]) : TC(plane<TS>((void*)src, P, kQ + d)[oo]);
-      if (type_o == 0) {
-        s12<I>(ro, hdt, fv, hv, fe, he, f2, h2);
-      } else {
-        f2 = fv;
-        h2 = hv;
-      }
-    }
-    if constexpr (I != 0) {
-      S.f[ly + kH][lx + kH] = f2;
-      S.h[ly + kH][lx + kH] = h2;
-      constexpr int k0 = EX ? 0 : kNXB;
-      constexpr int k1 = EY ? kNHalo : kNXB;
-      for (int k = k0 + (int)threadIdx.x; k < k1; k += kNT) {
-        int hx, hy;
-        halo_pos(k, hx, hy);
-        const unsigned mh = R.meta[hy + kH][hx + kH];
-        const long long oh = R.off[hy + kH][hx + kH];
-        const int d = remap_dir(I, mh);
-        const TC fv = TC(plane<TS>((void*)src, P, d)[oh]);
-        const TC hv = TC(plane<TS>((void*)src, P, kQ + d)[oh]);
-        TC a = fv, b = hv;
-        if ((mh & kMetaType) == 0) {
-          TC fe_, he_;
-          s12<I>(HS.rec[k], hdt, fv, hv, fe_, he_, a, b);
+    __syncthreads();
+    if (fluid) {
+      // reference cFLUX (cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) +
+      // (dt/dy)(Fy_out - Fy_in), grouped out - in; an upwind owner across a
+      // clamped edge is the cell itself
+      TC oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
+      if (ex) {
+        const int lo = sx > 0 ? -1 : 0;
+        oxf = S.fxf[ty][tx - lo];
+        oxh = S.fxh[ty][tx - lo];
+        const int ux = tx - sx;
+        if (R.meta[ty + kH][ux + kH] & kMetaClampX) {
+          ixf = oxf;
+          ixh = oxh;
+        } else {
+          ixf = S.fxf[ty][ux - lo];
+          ixh = S.fxh[ty][ux - lo];
         }
-        S.f[hy + kH][hx + kH] = a;
-        S.h[hy + kH][hx + kH] = b;
       }
-      __syncthreads();
-      tile_faces<I, TC>(S, R, in, solids);
-      __syncthreads();
+      if (ey) {
+        const int lo = sy > 0 ? -1 : 0;
+        oyf = S.fyf[ty - lo][tx];
+        oyh = S.fyh[ty - lo][tx];
+        const int uy = ty - sy;
+        if (R.meta[uy + kH][tx + kH] & kMetaClampY) {
+          iyf = oyf;
+          iyh = oyh;
+        } else {
+          iyf = S.fyf[uy - lo][tx];
+          iyh = S.fyh[uy - lo][tx];
+        }
+      }
+      TC dF, dH;
+      if (ex && ey) {
+        dF = (rdx * oxf + rdy * oyf) - (rdx * ixf + rdy * iyf);
+        dH = (rdx * oxh + rdy * oyh) - (rdx * ixh + rdy * iyh);
+      } else if (ex) {
+        dF = rdx * oxf - rdx * ixf;
+        dH = rdx * oxh - rdx * ixh;
+      } else {
+        dF = rdy * oyf - rdy * iyf;
+        dH = rdy * oyh - rdy * iyh;
+      }
+      if (MODE == 0) {
+        dst[P * (I - 1) + oo] = TS(dF);
+        dst[P * (12 + I - 1) + oo] = TS(dH);
+      } else {
+        TS* pf = dst + P * I + oo;
+        TS* ph = dst + P * (kQ + I) + oo;
+        *pf = TS(TC(*pf) - TC(0.5) * dF);
+        *ph = TS(TC(*ph) - TC(0.5) * dH);
+      }
     }
-    if (fluid) {
-      TC dF = 0, dH = 0;
-      if constexpr (I != 0) own_net_flux<I, TC>(S, R, in, lx, ly, dF, dH);
-      const TC fv = TC(fr[I]), hv = TC(hr[I]);
-      const TC D = fe - f2;
-      const TC eu2 = TC(lat_x(I)) * ro.cU + TC(lat_y(I)) * ro.cV;
-      const TC A2 = (eu2 / thf2) * D;
-      Pf[I] = (fv - dF) + af2 * D;
-      Qh[I] = ((hv - dH) - hdt * A2) + ah2 * (he - h2);
-      const TC gf = (fv - TC(0.5) * dF) + hdt * (D / tf2);
-      const TC gh = ((hv - TC(0.5) * dH) + hdt * ((he - h2) / th2)) - hdt * A2;
-      out[P * I + oo] = TS(gf);
-      out[P * (kQ + I) + oo] = TS(gh);
-      const double fd = fr[I], dd = double(dF);
-      const double exc = lat_x(I) * in.c, eyc = lat_y(I) * in.c;
-      rho3 += fd - dd;
-      rux3 += fd * exc - dd * exc;
-      ruy3 += fd * eyc - dd * eyc;
-      sh3 += hr[I] - double(dH);
-    } else if (own_in) {
-      // frozen cell: f3 = f2 = f and the new state is f
-      const TS a = src[P * I + oo], b = src[P * (kQ + I) + oo];
-      out[P * I + oo] = a;
-      out[P * (kQ + I) + oo] = b;
-      f3buf[P * I + oo] = a;
-      f3buf[P * (kQ + I) + oo] = b;
-    }
-  };
-  (one_dir(std::integral_constant<int, Is>{}), ...);
-
-  if (!fluid) return;
-  Macro m3;
-  finish_macro(in, rho3, rux3, ruy3, sh3, m3);
-  const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
-  const TC tf3 = TC(m3.tf), th3 = TC(m3.th), thf3 = TC(m3.thf);
-  const TC af3 = dt / (TC(2) * tf3), ah3 = dt / (TC(2) * th3);
-  const TC cU3 = TC(in.c) * TC(m3.ux), cV3 = TC(in.c) * TC(m3.uy);
-  double sumfe = 0;
-  auto finish = [&](auto Ic) {
-    constexpr int I = decltype(Ic)::value;
-    TC fe, he;
-    eq_dir<I>(q3, fe, he);
-    sumfe += double(fe);
-    const TC f3 = (Pf[I] + af3 * fe) / (TC(1) + af3);
-    const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
-    const TC A3 = (eu3 / thf3) * (fe - f3);
-    const TC h3 = ((Qh[I] - hdt * A3) + ah3 * he) / (TC(1) + ah3);
-    f3buf[P * I + oo] = TS(f3);
-    f3buf[P * (kQ + I) + oo] = TS(h3);
-  };
-  (finish(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m3.rho, sumfe);
+  }
 }
 
-template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) void k_stageA(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                                int step) {
+template <typename TS, typename TC, int MODE>
+__global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                              int step) {
   __shared__ Region R;
-  __shared__ Stage<TC> S;
-  __shared__ HaloStore<TC> HS;
+  __shared__ FluxStage<TC> S;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -645,94 +603,20 @@ __global__ __launch_bounds__(kNT) void k_stageA(const InstDev* insts, const Tile
     __syncthreads();
     setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
     __syncthreads();
-    stageA_tile<TS, TC>(S, HS, R, in, tr, par, step, MakeSeq<kQ>::type{});
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_stageB: the rest of the final combination (cpuLBM.cpp:1995-2042): the
-// stage-3 relaxation terms (equilibrium from the moments of f3, which equal
-// the moments of f - dF(f2)) and -dF(f3)/2.
-// ---------------------------------------------------------------------------
-template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void stageB_tile(Stage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
-                                            int par, Seq<Is...>) {
-  const int lx = (int)threadIdx.x % kTX, ly = (int)threadIdx.x / kTX;
-  const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
-  const bool own_in = x < in.nx && y < in.ny;
-  const long long oo = R.off[ly + kH][lx + kH];
-  const bool fluid = own_in && (R.meta[ly + kH][lx + kH] & kMetaType) == 0;
-  const bool solids = tr.flags & kTileSolid;
-  const long long P = in.plane;
-  const TS* f3b = reinterpret_cast<const TS*>(in.f3);
-  TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
-  const TC dt = TC(in.dt), hdt = dt / TC(2);
-  Macro m3{};
-  m3.rho = 1;
-  m3.tf = m3.th = m3.thf = 1;
-  if (fluid) {
-    double f[kQ], h[kQ];
-    load26(f3b, P, oo, f, h);
-    moments26(in, f, h, m3);
-  }
-  const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
-  const TC tf3 = TC(m3.tf), th3 = TC(m3.th), thf3 = TC(m3.thf);
-  const TC cU3 = TC(in.c) * TC(m3.ux), cV3 = TC(in.c) * TC(m3.uy);
-  auto one_dir = [&](auto Ic) {
-    constexpr int I = decltype(Ic)::value;
-    TC dF = 0, dH = 0;
-    if constexpr (I != 0) {
-      stage_raw<I, TS, TC>(S, R, in, in.f3);
-      __syncthreads();
-      tile_faces<I, TC>(S, R, in, solids);
-      __syncthreads();
-      if (fluid) own_net_flux<I, TC>(S, R, in, lx, ly, dF, dH);
-    }
-    if (fluid) {
-      TC fe, he;
-      eq_dir<I>(q3, fe, he);
-      const TC f3 = TC(f3b[P * I + oo]), h3 = TC(f3b[P * (kQ + I) + oo]);
-      const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
-      const TC A3 = (eu3 / thf3) * (fe - f3);
-      TS* pf = plane<TS>(out, P, I);
-      TS* ph = plane<TS>(out, P, kQ + I);
-      pf[oo] = TS((TC(pf[oo]) + hdt * ((fe - f3) / tf3)) - TC(0.5) * dF);
-      ph[oo] = TS(((TC(ph[oo]) + hdt * ((he - h3) / th3)) - hdt * A3) - TC(0.5) * dH);
-    }
-  };
-  (one_dir(std::integral_constant<int, Is>{}), ...);
-}
-
-template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) void k_stageB(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                                int step) {
-  __shared__ Region R;
-  __shared__ Stage<TC> S;
-  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
-    const TileRef tr = tiles[t];
-    const InstDev& in = insts[tr.inst];
-    if (frozen_by_failure(in, step)) continue;
-    __syncthreads();
-    setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
-    __syncthreads();
-    stageB_tile<TS, TC>(S, R, in, tr, par, MakeSeq<kQ>::type{});
+    flux_tile<TS, TC, MODE>(S, R, in, tr, par);
   }
 }
 
 // ---------------------------------------------------------------------------
 // k_alpha: wall closure factor of each wall cell with a non-solid axis
 // neighbour, from the neighbours' face fluxes of `which` (0: f2, 1: f3),
-// following the reference's physical stencil logic (cpuLBM.cpp:1171-1326,
+// following the reference's physical stencil addressing (cpuLBM.cpp:1171-1326,
 // 1502-1590).
 // ---------------------------------------------------------------------------
 __constant__ int c_wall_dx[4] = {-1, 1, 0, 0};
 __constant__ int c_wall_dy[4] = {0, 0, -1, 1};
 __constant__ int c_wall_in[4][4] = {{1, 5, 8, 9}, {3, 6, 7, 11}, {2, 5, 6, 10}, {4, 7, 8, 12}};
 __constant__ int c_wall_out[4][4] = {{3, 6, 7, 11}, {1, 5, 8, 9}, {4, 8, 7, 12}, {2, 6, 5, 10}};
-__constant__ int c_lat_x[13] = {0, 1, 0, -1, 0, 1, -1, -1, 1, 2, 0, -2, 0};
-__constant__ int c_lat_y[13] = {0, 0, 1, 0, -1, 1, 1, -1, -1, 0, 2, 0, -2};
-__constant__ int c_mir_ns[13] = {0, 1, 4, 3, 2, 8, 7, 6, 5, 9, 12, 11, 10};
-__constant__ int c_mir_ew[13] = {0, 3, 2, 1, 4, 6, 5, 8, 7, 11, 10, 9, 12};
 
 __device__ __forceinline__ int ref_index(const InstDev& in, int i, int pm, int d, int& mir) {
   unsigned m, cl;
@@ -819,8 +703,7 @@ __global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, 
 // macroscopic output (macroPropShort, cpuLBM.cpp:877-914).
 // ---------------------------------------------------------------------------
 template <typename TS, int... Is>
-__device__ __forceinline__ void init_cell(const InstDev& in, TS* dst, long long o, const double* pr,
-                                          Seq<Is...>) {
+__device__ __forceinline__ void init_cell(const InstDev& in, TS* dst, long long o, const double* pr, Seq<Is...>) {
   Macro m;
   m.rho = pr[0];
   m.ux = pr[1];
@@ -837,13 +720,12 @@ __device__ __forceinline__ void init_cell(const InstDev& in, TS* dst, long long 
   (one(std::integral_constant<int, Is>{}), ...);
 }
 
-// props: per instance, rows of {rho, ux, uy, T}; ids: [ny][pitch]
+// props: rows of {rho, ux, uy, T}; ids: [ny][pitch]
 template <typename TS>
 __global__ void k_init(const InstDev* insts, int inst, const int* ids, const double* props) {
   const InstDev& in = insts[inst];
   const long long n = (long long)in.ny * in.pitch;
-  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n;
-       o += (long long)gridDim.x * blockDim.x) {
+  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(o % in.pitch);
     if (x >= in.nx) continue;
     TS* dst = reinterpret_cast<TS*>(in.state[in.parity0]);
@@ -857,21 +739,13 @@ __global__ void k_macro(const InstDev* insts, int inst, int par, double* outp) {
   const InstDev& in = insts[inst];
   const long long n = (long long)in.ny * in.pitch;
   const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
-  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n;
-       o += (long long)gridDim.x * blockDim.x) {
+  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(o % in.pitch);
     if (x >= in.nx) continue;
-    double rho = 0, rux = 0, ruy = 0, sh = 0;
-#pragma unroll
-    for (int i = 0; i < kQ; i++) {
-      const double fi = double(src[in.plane * i + o]);
-      rho += fi;
-      rux += fi * (lat_x(i) * in.c);
-      ruy += fi * (lat_y(i) * in.c);
-      sh += double(src[in.plane * (kQ + i) + o]);
-    }
+    double f[kQ], h[kQ];
+    load26(src, in.plane, o, f, h);
     Macro m;
-    finish_macro(in, rho, rux, ruy, sh, m);
+    moments26(in, f, h, m);
     outp[o] = m.rho;
     outp[n + o] = m.ux;
     outp[2 * n + o] = m.uy;
@@ -886,45 +760,52 @@ __global__ void k_macro(const InstDev* insts, int inst, int par, double* outp) {
 int tile_w() { return kTX; }
 int tile_h() { return kTY; }
 int tile_halo() { return kH; }
+int dflux_planes() { return kNDF; }
 
-template <typename TS, typename TC>
-static int grid_for(int ntiles, const void* kernel) {
+static int grid_for(const void* kernel) {
   int dev = 0, cus = 256, per = 4;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kNT, 0) != hipSuccess || per < 1) per = 1;
   long long g = (long long)cus * per;
-  if (g > ntiles) g = ntiles;
-  if (g >= 8) g -= g % 8;
-  return (int)(g < 1 ? 1 : g);
+  g -= g % 8;
+  return (int)(g < 8 ? 8 : g);
 }
+
+static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 
 template <typename TS, typename TC>
 void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
-  static int gA = 0, gB = 0;
-  if (!gA) gA = grid_for<TS, TC>(1 << 30, (const void*)k_stageA<TS, TC>);
-  if (!gB) gB = grid_for<TS, TC>(1 << 30, (const void*)k_stageB<TS, TC>);
+  static int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+  if (!g0) g0 = grid_for((const void*)k_s12<TS, TC>);
+  if (!g1) g1 = grid_for((const void*)k_flux<TS, TC, 0>);
+  if (!g2) g2 = grid_for((const void*)k_s3<TS, TC>);
+  if (!g3) g3 = grid_for((const void*)k_flux<TS, TC, 1>);
   const int n = L.ntiles;
+  const int wb = (L.nwalls + 255) / 256;
+  if (tm) tm->begin(0, s);
+  k_s12<TS, TC><<<blocks_for(n, g0), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  if (tm) tm->end(0, s);
   if (L.nwalls) {
-    if (tm) tm->begin(0, s);
-    k_s12_list<TS, TC><<<(L.ncells + 255) / 256, 256, 0, s>>>(L.insts, L.cells, L.ncells, par, step);
-    if (tm) tm->end(0, s);
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<(L.nwalls + 255) / 256, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, step);
+    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  k_stageA<TS, TC><<<n < gA ? n : gA, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  k_flux<TS, TC, 0><<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
+  if (tm) tm->begin(3, s);
+  k_s3<TS, TC><<<blocks_for(n, g2), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  if (tm) tm->end(3, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<(L.nwalls + 255) / 256, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, step);
+    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, step);
     if (tm) tm->end(1, s);
   }
-  if (tm) tm->begin(3, s);
-  k_stageB<TS, TC><<<n < gB ? n : gB, kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
-  if (tm) tm->end(3, s);
+  if (tm) tm->begin(4, s);
+  k_flux<TS, TC, 1><<<blocks_for(n, g3), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  if (tm) tm->end(4, s);
 }
 
 template <typename TS>

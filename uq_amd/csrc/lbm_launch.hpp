@@ -19,8 +19,6 @@ struct StepLaunch {
   int ntiles = 0;
   const WallRef* walls = nullptr;
   int nwalls = 0;
-  const WallRef* cells = nullptr;  // cells whose f2 the wall closure reads
-  int ncells = 0;
 };
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
@@ -72,6 +70,7 @@ class KernelTimer {
 int tile_w();
 int tile_h();
 int tile_halo();
+int dflux_planes();
 
 template <typename TS, typename TC>
 void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm);

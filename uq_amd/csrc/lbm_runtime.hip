@@ -280,9 +280,19 @@ static int validate_geometry(const Solver& s) {
 // ---------------------------------------------------------------------------
 static size_t elem_size(int precision) { return precision == LBM_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
+// Device bytes of one solver: 4 x 26 distribution planes (two state buffers,
+// f2, f3), the dF planes, cell types, wall factor, property ids and the
+// macroscopic output planes.
+size_t uqlbm::solver_footprint(int nx, int ny, int precision) {
+  const size_t pitch = (size_t)((nx + 31) / 32 * 32);
+  const size_t plane = pitch * ny;
+  const size_t es = elem_size(precision);
+  return plane * (kPlanes * 4 + dflux_planes()) * es + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
+}
+
 Solver::~Solver() {
   for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
-                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)cells_dev, (void*)macro_dev})
+                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -305,6 +315,7 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.state[1] = s.state[1];
   d.f2 = s.f2;
   d.f3 = s.f3;
+  d.dflux = s.dflux;
   d.ctype = s.ctype;
   d.alpha = s.alpha;
   d.fail = s.fail;
@@ -325,8 +336,7 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.Pr = s.p.Pr;
 }
 
-static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls,
-                        std::vector<WallRef>& cells, int inst) {
+static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls, int inst) {
   const int TW = tile_w(), TH = tile_h(), H = tile_halo();
   const int tx_n = (s.nx + TW - 1) / TW, ty_n = (s.ny + TH - 1) / TH;
   // prefix count of solid cells for the per-tile solid flag
@@ -367,31 +377,6 @@ static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vecto
       }
       if (need) walls.push_back(WallRef{inst, x, y});
     }
-  // cells the closure reads: each wall cell and the 5-point stencil lines
-  // through its non-solid axis neighbours
-  std::vector<char> mark((size_t)s.nx * s.ny, 0);
-  auto add = [&](int x, int y) {
-    const size_t k = (size_t)y * s.nx + x;
-    if (!mark[k]) {
-      mark[k] = 1;
-      cells.push_back(WallRef{inst, x, y});
-    }
-  };
-  static const int dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};
-  for (const WallRef& w : walls) {
-    add(w.x, w.y);
-    for (int j = 0; j < 4; j++) {
-      int m;
-      bool cl;
-      const int xn = ref_map(s, w.x + dxs[j], 0, m, cl);
-      const int yn = ref_map(s, w.y + dys[j], 1, m, cl);
-      if (type_at(s, xn, yn) >= 2) continue;
-      for (int k = -2; k <= 2; k++) {
-        if (j < 2) add(ref_map(s, xn + k, 0, m, cl), yn);
-        else add(xn, ref_map(s, yn + k, 1, m, cl));
-      }
-    }
-  }
 }
 
 }  // namespace uqlbm
@@ -461,6 +446,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   const size_t es = elem_size(precision);
   HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   for (void** b : {&s->state[0], &s->state[1], &s->f2, &s->f3}) HIP_TRY(hipMalloc(b, plane * kPlanes * es));
+  HIP_TRY(hipMalloc(&s->dflux, plane * dflux_planes() * es));
   HIP_TRY(hipMalloc((void**)&s->ctype, plane));
   HIP_TRY(hipMalloc((void**)&s->alpha, plane * sizeof(double)));
   HIP_TRY(hipMalloc((void**)&s->fail, sizeof(int)));
@@ -468,7 +454,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   HIP_TRY(hipMalloc((void**)&s->props_dev, s->props.size() * 4 * sizeof(double)));
   HIP_TRY(hipMalloc((void**)&s->inst_dev, sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&s->macro_dev, plane * 5 * sizeof(double)));
-  s->device_bytes = plane * kPlanes * es * 4 + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
+  s->device_bytes = solver_footprint(nx, ny, precision);
 
   std::vector<uint8_t> ct(plane, 2);
   std::vector<int> ids(plane, 0);
@@ -489,26 +475,21 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   HIP_TRY(hipMemcpy(s->props_dev, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(s->alpha, 0, plane * sizeof(double)));
   for (void* b : {s->state[0], s->state[1], s->f2, s->f3}) HIP_TRY(hipMemset(b, 0, plane * kPlanes * es));
+  HIP_TRY(hipMemset(s->dflux, 0, plane * dflux_planes() * es));
 
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls, cells;
-  build_lists(*s, tiles, walls, cells, 0);
+  std::vector<WallRef> walls;
+  build_lists(*s, tiles, walls, 0);
   s->ntiles = (int)tiles.size();
   s->nwalls = (int)walls.size();
-  s->ncells = (int)cells.size();
   HIP_TRY(hipMalloc((void**)&s->tiles_dev, tiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(s->tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   if (!walls.empty()) {
     HIP_TRY(hipMalloc((void**)&s->walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(s->walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
-  if (!cells.empty()) {
-    HIP_TRY(hipMalloc((void**)&s->cells_dev, cells.size() * sizeof(WallRef)));
-    HIP_TRY(hipMemcpy(s->cells_dev, cells.data(), cells.size() * sizeof(WallRef), hipMemcpyHostToDevice));
-  }
   s->tiles_host = std::move(tiles);
   s->walls_host = std::move(walls);
-  s->cells_host = std::move(cells);
   st = upload_inst(*s);
   if (st != LBM_OK) return st;
   *out = reinterpret_cast<lbm_solver*>(s.release());
@@ -566,7 +547,7 @@ static KernelTimer* active_timer() {
   if (!g_timing_enabled) return nullptr;
   if (!g_timer) {
     g_timer = new KernelTimer();
-    g_timer->set_names({"s12_walls", "alpha", "stageA", "stageB"});
+    g_timer->set_names({"s12", "alpha", "flux2", "s3", "flux3"});
   }
   return g_timer;
 }
@@ -636,7 +617,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
   if (s.failed) return LBM_FAILED;
   if (nsteps == 0) return LBM_OK;
-  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls, s.cells_dev, s.ncells};
+  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls};
   std::vector<Solver*> one{&s};
   int st = LBM_OK;
   if (want_residual && nsteps > 1) {
@@ -800,7 +781,7 @@ extern "C" int lbm_batch_create(lbm_solver* const* members, int32_t n, lbm_batch
 
 int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls, cells;
+  std::vector<WallRef> walls;
   for (size_t k = 0; k < b.members.size(); k++) {
     for (TileRef t : b.members[k]->tiles_host) {
       t.inst = (int)k;
@@ -810,14 +791,9 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
       w.inst = (int)k;
       walls.push_back(w);
     }
-    for (WallRef w : b.members[k]->cells_host) {
-      w.inst = (int)k;
-      cells.push_back(w);
-    }
   }
   b.ntiles = (int)tiles.size();
   b.nwalls = (int)walls.size();
-  b.ncells = (int)cells.size();
   HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
@@ -826,22 +802,18 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
     HIP_TRY(hipMalloc((void**)&b.walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(b.walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
-  if (!cells.empty()) {
-    HIP_TRY(hipMalloc((void**)&b.cells_dev, cells.size() * sizeof(WallRef)));
-    HIP_TRY(hipMemcpy(b.cells_dev, cells.data(), cells.size() * sizeof(WallRef), hipMemcpyHostToDevice));
-  }
   *out = reinterpret_cast<lbm_batch*>(new Batch(std::move(b)));
   return LBM_OK;
 }
 
 uqlbm::Batch::~Batch() {
-  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)cells_dev})
+  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 int uqlbm::batch_step(Batch& b, int nsteps) {
-  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls, b.cells_dev, b.ncells};
+  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls};
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

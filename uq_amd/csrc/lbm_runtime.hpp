@@ -31,12 +31,12 @@ struct Solver {
   std::vector<uint8_t> ctype_host;  // [y][nx]
   std::vector<TileRef> tiles_host;
   std::vector<WallRef> walls_host;
-  std::vector<WallRef> cells_host;  // stencil cells of the wall closure
 
   hipStream_t stream = nullptr;
   void* state[2] = {nullptr, nullptr};
   void* f2 = nullptr;
   void* f3 = nullptr;
+  void* dflux = nullptr;
   uint8_t* ctype = nullptr;
   double* alpha = nullptr;
   int* fail = nullptr;
@@ -46,9 +46,8 @@ struct Solver {
   InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  WallRef* cells_dev = nullptr;
   double* macro_dev = nullptr;
-  int ntiles = 0, nwalls = 0, ncells = 0;
+  int ntiles = 0, nwalls = 0;
   size_t device_bytes = 0;
 
   int cur = 0;    // state buffer holding the current state
@@ -69,8 +68,7 @@ struct Batch {
   InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  WallRef* cells_dev = nullptr;
-  int ntiles = 0, nwalls = 0, ncells = 0;
+  int ntiles = 0, nwalls = 0;
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -81,21 +79,19 @@ struct Batch {
     inst_dev = o.inst_dev;
     tiles_dev = o.tiles_dev;
     walls_dev = o.walls_dev;
-    cells_dev = o.cells_dev;
     ntiles = o.ntiles;
     nwalls = o.nwalls;
-    ncells = o.ncells;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;
     o.walls_dev = nullptr;
-    o.cells_dev = nullptr;
     return *this;
   }
   ~Batch();
 };
 
 int fail_with(int status, const char* fmt, ...);
+size_t solver_footprint(int nx, int ny, int precision);
 int upload_inst(Solver& s);
 int batch_finish(Batch& b, lbm_batch** out);
 int batch_step(Batch& b, int nsteps);

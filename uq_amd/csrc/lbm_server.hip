@@ -61,9 +61,8 @@ struct Server {
   InstDev* table_dev = nullptr;
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  WallRef* cells_dev = nullptr;
-  size_t tiles_cap = 0, walls_cap = 0, cells_cap = 0;
-  int ntiles = 0, nwalls = 0, ncells = 0;
+  size_t tiles_cap = 0, walls_cap = 0;
+  int ntiles = 0, nwalls = 0;
   bool lists_dirty = true;
   ~Server();
 };
@@ -73,17 +72,11 @@ static double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Device bytes a solver of this size will hold (matches lbm_create).
-static size_t footprint_of(int nx, int ny, int precision) {
-  const size_t pitch = (size_t)((nx + 31) / 32 * 32);
-  const size_t plane = pitch * ny;
-  const size_t es = precision == LBM_PREC_FP64 ? 8 : 4;
-  return plane * kPlanes * es * 4 + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
-}
+static size_t footprint_of(int nx, int ny, int precision) { return solver_footprint(nx, ny, precision); }
 
 Server::~Server() {
   for (auto& a : active) delete a.solver;
-  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)cells_dev})
+  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -96,7 +89,7 @@ Server::~Server() {
 
 static int rebuild_lists(Server& sv) {
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls, cells;
+  std::vector<WallRef> walls;
   for (size_t k = 0; k < sv.active.size(); k++) {
     for (TileRef t : sv.active[k].solver->tiles_host) {
       t.inst = (int)k;
@@ -105,10 +98,6 @@ static int rebuild_lists(Server& sv) {
     for (WallRef w : sv.active[k].solver->walls_host) {
       w.inst = (int)k;
       walls.push_back(w);
-    }
-    for (WallRef w : sv.active[k].solver->cells_host) {
-      w.inst = (int)k;
-      cells.push_back(w);
     }
   }
   if (tiles.size() > sv.tiles_cap) {
@@ -121,14 +110,6 @@ static int rebuild_lists(Server& sv) {
     sv.walls_cap = std::max(walls.size(), sv.walls_cap * 2);
     SV_TRY(hipMalloc((void**)&sv.walls_dev, sv.walls_cap * sizeof(WallRef)));
   }
-  if (cells.size() > sv.cells_cap) {
-    if (sv.cells_dev) SV_TRY(hipFree(sv.cells_dev));
-    sv.cells_cap = std::max(cells.size(), sv.cells_cap * 2);
-    SV_TRY(hipMalloc((void**)&sv.cells_dev, sv.cells_cap * sizeof(WallRef)));
-  }
-  if (!cells.empty())
-    SV_TRY(hipMemcpyAsync(sv.cells_dev, cells.data(), cells.size() * sizeof(WallRef), hipMemcpyHostToDevice,
-                          sv.stream));
   if (!tiles.empty())
     SV_TRY(hipMemcpyAsync(sv.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
                           sv.stream));
@@ -138,7 +119,6 @@ static int rebuild_lists(Server& sv) {
   SV_TRY(hipStreamSynchronize(sv.stream));
   sv.ntiles = (int)tiles.size();
   sv.nwalls = (int)walls.size();
-  sv.ncells = (int)cells.size();
   sv.lists_dirty = false;
   return LBM_OK;
 }
@@ -285,7 +265,7 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     std::vector<Solver*> members;
     members.reserve(sv.active.size());
     for (auto& a : sv.active) members.push_back(a.solver);
-    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls, sv.cells_dev, sv.ncells};
+    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls};
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
     st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round);
