@@ -131,3 +131,12 @@ def test_periodic_y_mirror_x(gpu_lib):
     c.params["mirror_east"] = 1
     c.params["mirror_west"] = 1
     _compare(c, 12, gpu_lib)
+
+
+@pytest.mark.parametrize("rk", [0, 2, 3], ids=["RK1", "EULER", "EULER_EQ"])
+def test_one_stage_steppers(gpu_lib, rk):
+    """RKMETHOD 0/2/3 (reference RK1, EULER, EULER_EQ): Couette (walls),
+    a mirrored flat plate and a periodic shear wave."""
+    for c in (cases.couette(40, rk_method=rk), cases.shear_wave(32, rk_method=rk),
+              cases.flat_plate(nx=48, ny=32, plate_start=12, steps=10, rk_method=rk)):
+        _compare(c, 10, gpu_lib)

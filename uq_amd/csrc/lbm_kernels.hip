@@ -175,17 +175,18 @@ __device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC
        (TC(1) + r.ah);
 }
 
-// Own cell of a pointwise tile launch (threads with lx == 15 have none).
+// Own cell of a pointwise launch: 64 x 4 row tiles, so each wavefront reads
+// one aligned 64-cell row segment of a plane.
 struct PointCell {
   long long o;
   bool active;
 };
 
 __device__ __forceinline__ PointCell point_cell(const InstDev& in, const TileRef& tr) {
-  const int lx = (int)threadIdx.x % 16, ly = (int)threadIdx.x / 16;
-  const int x = tr.tx * kTX + lx, y = tr.ty * kTY + ly;
+  const int x = tr.tx * kPointW + (int)threadIdx.x % kPointW;
+  const int y = tr.ty * kPointH + (int)threadIdx.x / kPointW;
   PointCell c;
-  c.active = lx < kTX && x < in.nx && y < in.ny;
+  c.active = x < in.nx && y < in.ny;
   c.o = c.active ? (long long)y * in.pitch + x : 0;
   return c;
 }
@@ -346,6 +347,153 @@ __global__ __launch_bounds__(kNT) void k_s3(const InstDev* insts, const TileRef*
 }
 
 // ---------------------------------------------------------------------------
+// The one-flux-stage steppers (reference RK1 cpuLBM.cpp:1597-1737, EULER
+// :2104-2193, EULER_EQ :2194-2283):
+//   RK1:      k_s1 (f1 -> f2 buffer), k_flux<0>, k_fin<0>
+//   EULER:    k_flux<2>, k_fin<2>
+//   EULER_EQ: k_flux<2>, k_fin<3>
+// ---------------------------------------------------------------------------
+template <typename TC>
+struct Rec1 {
+  EqIn<TC> q;
+  TC cU, cV;
+  TC bf, bh, dt, ithf;  // dt/tf, dt/th, dt, 1/thf
+};
+
+template <typename TC>
+__device__ __forceinline__ Rec1<TC> make_rec1(const InstDev& in, const Macro& m) {
+  Rec1<TC> r;
+  r.q = eq_inputs<TC>(in, m);
+  r.cU = TC(in.c) * TC(m.ux);
+  r.cV = TC(in.c) * TC(m.uy);
+  r.dt = TC(in.dt);
+  r.bf = r.dt / TC(m.tf);
+  r.bh = r.dt / TC(m.th);
+  r.ithf = TC(1) / TC(m.thf);
+  return r;
+}
+
+// implicit stage of RK1 (cpuLBM.cpp:1650, 1660)
+template <int I, typename TC>
+__device__ __forceinline__ void s1(const Rec1<TC>& r, TC fv, TC hv, TC& fe, TC& he, TC& f1, TC& h1) {
+  eq_dir<I>(r.q, fe, he);
+  f1 = (fv + r.bf * fe) / (TC(1) + r.bf);
+  const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
+  h1 = (hv - r.dt * eu * ((fe - f1) * r.ithf) + r.bh * he) / (TC(1) + r.bh);
+}
+
+template <typename TS, typename TC, int... Is>
+__device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, TS* dst, long long o, int step,
+                                        Seq<Is...>) {
+  const long long P = in.plane;
+  double f[kQ], h[kQ];
+  load26(src, P, o, f, h);
+  Macro m;
+  moments26(in, f, h, m);
+  const Rec1<TC> r = make_rec1<TC>(in, m);
+  double sumfe = 0, sumabs = 0;
+  auto one = [&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    TC fe, he, f1, h1;
+    s1<I>(r, TC(f[I]), TC(h[I]), fe, he, f1, h1);
+    sumfe += double(fe);
+    sumabs += fabs(double(fe));
+    dst[P * I + o] = TS(f1);
+    dst[P * (kQ + I) + o] = TS(h1);
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+  flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
+}
+
+template <typename TS, typename TC>
+__global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                            int step) {
+  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const InstDev& in = insts[tr.inst];
+    if (frozen_by_failure(in, step)) continue;
+    const PointCell c = point_cell(in, tr);
+    if (!c.active) continue;
+    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+    TS* dst = reinterpret_cast<TS*>(in.f2);
+    if (in.ctype[c.o] > 0) {
+#pragma unroll
+      for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
+      continue;
+    }
+    s1_cell<TS, TC>(in, src, dst, c.o, step, MakeSeq<kQ>::type{});
+  }
+}
+
+// Final update of the one-flux-stage steppers, METHOD 0 (RK1, cpuLBM.cpp:
+// 1696, 1707), 2 (EULER, :2155, :2165), 3 (EULER_EQ, :2245, :2255).
+template <int METHOD, typename TS, typename TC, int... Is>
+__device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const TS* dfb, TS* out, long long o,
+                                         int step, Seq<Is...>) {
+  const long long P = in.plane;
+  double f[kQ], h[kQ];
+  load26(src, P, o, f, h);
+  Macro m;
+  moments26(in, f, h, m);
+  const Rec1<TC> r = make_rec1<TC>(in, m);
+  double sumfe = 0, sumabs = 0;
+  auto one = [&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    TC dF = 0, dH = 0;
+    if constexpr (I != 0) {
+      dF = TC(dfb[P * (I - 1) + o]);
+      dH = TC(dfb[P * (12 + I - 1) + o]);
+    }
+    const TC fv = TC(f[I]), hv = TC(h[I]);
+    const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
+    TC nf, nh;
+    if constexpr (METHOD == 0) {
+      TC fe, he, f1, h1;
+      s1<I>(r, fv, hv, fe, he, f1, h1);
+      nf = fv - dF + r.bf * (fe - f1);
+      nh = hv - dH + r.bh * (he - h1) - ((r.dt * eu) * r.ithf) * (fe - f1);
+    } else {
+      TC fe, he;
+      eq_dir<I>(r.q, fe, he);
+      sumfe += double(fe);
+      sumabs += fabs(double(fe));
+      if constexpr (METHOD == 2) {
+        nf = fv - dF + r.bf * (fe - fv);
+        nh = hv - dH + r.bh * (he - hv) - ((r.dt * eu) * r.ithf) * (fe - fv);
+      } else {
+        nf = fe - dF;
+        nh = he - dH;
+      }
+    }
+    out[P * I + o] = TS(nf);
+    out[P * (kQ + I) + o] = TS(nh);
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+  if (METHOD != 0) flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
+}
+
+template <typename TS, typename TC, int METHOD>
+__global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                             int step) {
+  for (int t = first_tile(); t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const InstDev& in = insts[tr.inst];
+    if (frozen_by_failure(in, step)) continue;
+    const PointCell c = point_cell(in, tr);
+    if (!c.active) continue;
+    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+    TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+    if (in.ctype[c.o] > 0) {
+#pragma unroll
+      for (int k = 0; k < kPlanes; k++) out[in.plane * k + c.o] = src[in.plane * k + c.o];
+      continue;
+    }
+    fin_cell<METHOD, TS, TC>(in, src, reinterpret_cast<const TS*>(in.dflux), out, c.o, step,
+                             MakeSeq<kQ>::type{});
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stencil tiles: region (tile + 3-cell halo cross), staging and faces.
 // ---------------------------------------------------------------------------
 struct Region {
@@ -474,8 +622,9 @@ __device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& 
 
 // ---------------------------------------------------------------------------
 // k_flux<MODE>: net flux of every direction for the tile's cells.
-//   MODE 0: source f2; dF, dH -> the dF planes.
+//   MODE 0: source f2 (f1 for RK1); dF, dH -> the dF planes.
 //   MODE 1: source f3; new state (holding G) -= dF/2, dH/2.
+//   MODE 2: source the current state (EULER, EULER_EQ); dF, dH -> dF planes.
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, int MODE>
 __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
@@ -488,8 +637,8 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
   const long long oo = own ? R.off[ty + kH][tx + kH] : 0;
   const bool solids = tr.flags & kTileSolid;
   const long long P = in.plane;
-  void* src = MODE == 0 ? in.f2 : in.f3;
-  TS* dst = MODE == 0 ? reinterpret_cast<TS*>(in.dflux) : reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+  void* src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
+  TS* dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
   const TC rdx = TC(in.rdx), rdy = TC(in.rdy);
   for (int I = 1; I < kQ; I++) {
     const int ex = c_lat_x[I], ey = c_lat_y[I];
@@ -578,7 +727,7 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
         dF = rdy * oyf - rdy * iyf;
         dH = rdy * oyh - rdy * iyh;
       }
-      if (MODE == 0) {
+      if (MODE != 1) {
         dst[P * (I - 1) + oo] = TS(dF);
         dst[P * (12 + I - 1) + oo] = TS(dH);
       } else {
@@ -609,7 +758,8 @@ __global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRe
 
 // ---------------------------------------------------------------------------
 // k_alpha: wall closure factor of each wall cell with a non-solid axis
-// neighbour, from the neighbours' face fluxes of `which` (0: f2, 1: f3),
+// neighbour, from the neighbours' face fluxes of `which` (0: f2 or f1,
+// 1: f3, 2: the current state),
 // following the reference's physical stencil addressing (cpuLBM.cpp:1171-1326,
 // 1502-1590).
 // ---------------------------------------------------------------------------
@@ -669,13 +819,14 @@ __device__ TC ref_face_flux_f(const InstDev& in, const TS* src, int x, int y, in
 }
 
 template <typename TS, typename TC>
-__global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, int which, int step) {
+__global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, int which, int par, int step) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwalls) return;
   const WallRef wr = walls[w];
   const InstDev& in = insts[wr.inst];
   if (frozen_by_failure(in, step)) return;
-  const TS* src = reinterpret_cast<const TS*>(which ? in.f3 : in.f2);
+  const void* srcv = which == 0 ? in.f2 : (which == 1 ? in.f3 : in.state[in.parity0 ^ par]);
+  const TS* src = reinterpret_cast<const TS*>(srcv);
   const long long ow = (long long)wr.y * in.pitch + wr.x;
   double in_f = 0, out_f = 0;
   for (int j = 0; j < 4; j++) {
@@ -776,7 +927,7 @@ static int grid_for(const void* kernel) {
 static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 
 template <typename TS, typename TC>
-void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
+static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
   static int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
   if (!g0) g0 = grid_for((const void*)k_s12<TS, TC>);
   if (!g1) g1 = grid_for((const void*)k_flux<TS, TC, 0>);
@@ -785,27 +936,65 @@ void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, Kern
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
-  k_s12<TS, TC><<<blocks_for(n, g0), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  k_s12<TS, TC><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(0, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, step);
+    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
   k_flux<TS, TC, 0><<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_s3<TS, TC><<<blocks_for(n, g2), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  k_s3<TS, TC><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, step);
+    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(4, s);
   k_flux<TS, TC, 1><<<blocks_for(n, g3), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(4, s);
+}
+
+// RKMETHOD 0, 2, 3: one flux stage.
+template <typename TS, typename TC, int METHOD>
+static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
+  static int g0 = 0, g1 = 0, g2 = 0;
+  constexpr int FM = METHOD == 0 ? 0 : 2;
+  if (!g0) g0 = grid_for((const void*)k_s1<TS, TC>);
+  if (!g1) g1 = grid_for((const void*)k_flux<TS, TC, FM>);
+  if (!g2) g2 = grid_for((const void*)k_fin<TS, TC, METHOD>);
+  const int n = L.ntiles;
+  const int wb = (L.nwalls + 255) / 256;
+  if constexpr (METHOD == 0) {
+    if (tm) tm->begin(0, s);
+    k_s1<TS, TC><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+    if (tm) tm->end(0, s);
+  }
+  if (L.nwalls) {
+    if (tm) tm->begin(1, s);
+    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, METHOD == 0 ? 0 : 2, par, step);
+    if (tm) tm->end(1, s);
+  }
+  if (tm) tm->begin(2, s);
+  k_flux<TS, TC, FM><<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  if (tm) tm->end(2, s);
+  if (tm) tm->begin(3, s);
+  k_fin<TS, TC, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  if (tm) tm->end(3, s);
+}
+
+template <typename TS, typename TC>
+void launch_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
+  switch (L.rk_method) {
+    case 0: launch_one_stage<TS, TC, 0>(L, s, par, step, tm); break;
+    case 2: launch_one_stage<TS, TC, 2>(L, s, par, step, tm); break;
+    case 3: launch_one_stage<TS, TC, 3>(L, s, par, step, tm); break;
+    default: launch_rk3<TS, TC>(L, s, par, step, tm); break;
+  }
 }
 
 template <typename TS>
@@ -821,8 +1010,8 @@ void launch_macro(const InstDev* insts, int inst, int par, double* out, long lon
   k_macro<TS><<<blocks, 256, 0, s>>>(insts, inst, par, out);
 }
 
-template void launch_rk3_step<double, double>(const StepLaunch&, hipStream_t, int, int, KernelTimer*);
-template void launch_rk3_step<float, float>(const StepLaunch&, hipStream_t, int, int, KernelTimer*);
+template void launch_step<double, double>(const StepLaunch&, hipStream_t, int, int, KernelTimer*);
+template void launch_step<float, float>(const StepLaunch&, hipStream_t, int, int, KernelTimer*);
 template void launch_init<double>(const InstDev*, int, const int*, const double*, long long, hipStream_t);
 template void launch_init<float>(const InstDev*, int, const int*, const double*, long long, hipStream_t);
 template void launch_macro<double>(const InstDev*, int, int, double*, long long, hipStream_t);

@@ -19,6 +19,9 @@ struct StepLaunch {
   int ntiles = 0;
   const WallRef* walls = nullptr;
   int nwalls = 0;
+  const TileRef* ptiles = nullptr;  // 64 x 4 row tiles of the pointwise kernels
+  int nptiles = 0;
+  int rk_method = 1;                // RKMETHOD shared by every instance of the group
 };
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
@@ -71,9 +74,10 @@ int tile_w();
 int tile_h();
 int tile_halo();
 int dflux_planes();
+constexpr int kPointW = 64, kPointH = 4;  // pointwise tile: 64 x 4 cells, 256 threads
 
 template <typename TS, typename TC>
-void launch_rk3_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm);
+void launch_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm);
 
 template <typename TS>
 void launch_init(const InstDev* insts, int inst, const int* ids, const double* props, long long cells,

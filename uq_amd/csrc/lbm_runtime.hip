@@ -284,7 +284,7 @@ static size_t elem_size(int precision) { return precision == LBM_PREC_FP64 ? siz
 // f2, f3), the dF planes, cell types, wall factor, property ids and the
 // macroscopic output planes.
 size_t uqlbm::solver_footprint(int nx, int ny, int precision) {
-  const size_t pitch = (size_t)((nx + 31) / 32 * 32);
+  const size_t pitch = (size_t)((nx + 63) / 64 * 64);
   const size_t plane = pitch * ny;
   const size_t es = elem_size(precision);
   return plane * (kPlanes * 4 + dflux_planes()) * es + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
@@ -292,7 +292,7 @@ size_t uqlbm::solver_footprint(int nx, int ny, int precision) {
 
 Solver::~Solver() {
   for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
-                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux})
+                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux, (void*)ptiles_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -336,7 +336,10 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.Pr = s.p.Pr;
 }
 
-static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls, int inst) {
+static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls,
+                        std::vector<TileRef>& ptiles, int inst) {
+  for (int ty = 0; ty < (s.ny + kPointH - 1) / kPointH; ty++)
+    for (int tx = 0; tx < (s.nx + kPointW - 1) / kPointW; tx++) ptiles.push_back(TileRef{inst, tx, ty, 0});
   const int TW = tile_w(), TH = tile_h(), H = tile_halo();
   const int tx_n = (s.nx + TW - 1) / TW, ty_n = (s.ny + TH - 1) / TH;
   // prefix count of solid cells for the per-tile solid flag
@@ -403,8 +406,8 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   if (nx < 4 || ny < 4) return fail_with(LBM_ERR_ARG, "grid %dx%d too small (need at least 4x4)", nx, ny);
   if (precision != LBM_PREC_FP64 && precision != LBM_PREC_MIXED)
     return fail_with(LBM_ERR_ARG, "unknown precision %d", precision);
-  if (params->rk_method != 1)
-    return fail_with(LBM_ERR_ARG, "RKMETHOD %d not supported on the device path (only 1)", params->rk_method);
+  if (params->rk_method < 0 || params->rk_method > 3)
+    return fail_with(LBM_ERR_ARG, "RKMETHOD %d unknown (0 RK1, 1 RK3, 2 EULER, 3 EULER_EQ)", params->rk_method);
   if (params->flux_method != 0 && params->flux_method != 1)
     return fail_with(LBM_ERR_ARG, "FMETHOD %d unknown", params->flux_method);
   if (props->num_props < 1 || !props->density || !props->pressure || !props->temp || !props->velocity_x ||
@@ -440,7 +443,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   if (st != LBM_OK) return st;
   if (lbm_device_count() < 1) return fail_with(LBM_ERR_NOGPU, "no gfx950 device visible");
 
-  const int align = 32;
+  const int align = 64;
   s->pitch = (nx + align - 1) / align * align;
   const size_t plane = (size_t)s->pitch * ny;
   const size_t es = elem_size(precision);
@@ -479,9 +482,13 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
 
   std::vector<TileRef> tiles;
   std::vector<WallRef> walls;
-  build_lists(*s, tiles, walls, 0);
+  std::vector<TileRef> ptiles;
+  build_lists(*s, tiles, walls, ptiles, 0);
   s->ntiles = (int)tiles.size();
   s->nwalls = (int)walls.size();
+  s->nptiles = (int)ptiles.size();
+  HIP_TRY(hipMalloc((void**)&s->ptiles_dev, ptiles.size() * sizeof(TileRef)));
+  HIP_TRY(hipMemcpy(s->ptiles_dev, ptiles.data(), ptiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc((void**)&s->tiles_dev, tiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(s->tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   if (!walls.empty()) {
@@ -490,6 +497,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   }
   s->tiles_host = std::move(tiles);
   s->walls_host = std::move(walls);
+  s->ptiles_host = std::move(ptiles);
   st = upload_inst(*s);
   if (st != LBM_OK) return st;
   *out = reinterpret_cast<lbm_solver*>(s.release());
@@ -538,8 +546,8 @@ static void launch_steps(const StepLaunch& L, int precision, hipStream_t stream,
                          KernelTimer* tm) {
   for (int k = 0; k < nsteps; k++) {
     const int par = k & 1;
-    if (precision == LBM_PREC_FP64) launch_rk3_step<double, double>(L, stream, par, k + 1, tm);
-    else launch_rk3_step<float, float>(L, stream, par, k + 1, tm);
+    if (precision == LBM_PREC_FP64) launch_step<double, double>(L, stream, par, k + 1, tm);
+    else launch_step<float, float>(L, stream, par, k + 1, tm);
   }
 }
 
@@ -617,7 +625,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
   if (s.failed) return LBM_FAILED;
   if (nsteps == 0) return LBM_OK;
-  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls};
+  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls, s.ptiles_dev, s.nptiles, s.p.rk_method};
   std::vector<Solver*> one{&s};
   int st = LBM_OK;
   if (want_residual && nsteps > 1) {
@@ -782,10 +790,15 @@ extern "C" int lbm_batch_create(lbm_solver* const* members, int32_t n, lbm_batch
 int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   std::vector<TileRef> tiles;
   std::vector<WallRef> walls;
+  std::vector<TileRef> ptiles;
   for (size_t k = 0; k < b.members.size(); k++) {
     for (TileRef t : b.members[k]->tiles_host) {
       t.inst = (int)k;
       tiles.push_back(t);
+    }
+    for (TileRef t : b.members[k]->ptiles_host) {
+      t.inst = (int)k;
+      ptiles.push_back(t);
     }
     for (WallRef w : b.members[k]->walls_host) {
       w.inst = (int)k;
@@ -794,6 +807,7 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   }
   b.ntiles = (int)tiles.size();
   b.nwalls = (int)walls.size();
+  b.nptiles = (int)ptiles.size();
   HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
@@ -802,18 +816,20 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
     HIP_TRY(hipMalloc((void**)&b.walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(b.walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
+  HIP_TRY(hipMalloc((void**)&b.ptiles_dev, ptiles.size() * sizeof(TileRef)));
+  HIP_TRY(hipMemcpy(b.ptiles_dev, ptiles.data(), ptiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   *out = reinterpret_cast<lbm_batch*>(new Batch(std::move(b)));
   return LBM_OK;
 }
 
 uqlbm::Batch::~Batch() {
-  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev})
+  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 int uqlbm::batch_step(Batch& b, int nsteps) {
-  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls};
+  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls, b.ptiles_dev, b.nptiles, b.rk_method};
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

@@ -31,6 +31,7 @@ struct Solver {
   std::vector<uint8_t> ctype_host;  // [y][nx]
   std::vector<TileRef> tiles_host;
   std::vector<WallRef> walls_host;
+  std::vector<TileRef> ptiles_host;
 
   hipStream_t stream = nullptr;
   void* state[2] = {nullptr, nullptr};
@@ -46,8 +47,9 @@ struct Solver {
   InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
+  TileRef* ptiles_dev = nullptr;
   double* macro_dev = nullptr;
-  int ntiles = 0, nwalls = 0;
+  int ntiles = 0, nwalls = 0, nptiles = 0;
   size_t device_bytes = 0;
 
   int cur = 0;    // state buffer holding the current state
@@ -68,7 +70,8 @@ struct Batch {
   InstDev inst_host{};              // host image of *inst_dev (kept alive for async copies)
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  int ntiles = 0, nwalls = 0;
+  TileRef* ptiles_dev = nullptr;
+  int ntiles = 0, nwalls = 0, nptiles = 0;
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -79,12 +82,15 @@ struct Batch {
     inst_dev = o.inst_dev;
     tiles_dev = o.tiles_dev;
     walls_dev = o.walls_dev;
+    ptiles_dev = o.ptiles_dev;
     ntiles = o.ntiles;
     nwalls = o.nwalls;
+    nptiles = o.nptiles;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;
     o.walls_dev = nullptr;
+    o.ptiles_dev = nullptr;
     return *this;
   }
   ~Batch();

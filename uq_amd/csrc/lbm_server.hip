@@ -61,8 +61,9 @@ struct Server {
   InstDev* table_dev = nullptr;
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
-  size_t tiles_cap = 0, walls_cap = 0;
-  int ntiles = 0, nwalls = 0;
+  TileRef* ptiles_dev = nullptr;
+  size_t tiles_cap = 0, walls_cap = 0, ptiles_cap = 0;
+  int ntiles = 0, nwalls = 0, nptiles = 0;
   bool lists_dirty = true;
   ~Server();
 };
@@ -76,7 +77,7 @@ static size_t footprint_of(int nx, int ny, int precision) { return solver_footpr
 
 Server::~Server() {
   for (auto& a : active) delete a.solver;
-  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev})
+  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -90,10 +91,15 @@ Server::~Server() {
 static int rebuild_lists(Server& sv) {
   std::vector<TileRef> tiles;
   std::vector<WallRef> walls;
+  std::vector<TileRef> ptiles;
   for (size_t k = 0; k < sv.active.size(); k++) {
     for (TileRef t : sv.active[k].solver->tiles_host) {
       t.inst = (int)k;
       tiles.push_back(t);
+    }
+    for (TileRef t : sv.active[k].solver->ptiles_host) {
+      t.inst = (int)k;
+      ptiles.push_back(t);
     }
     for (WallRef w : sv.active[k].solver->walls_host) {
       w.inst = (int)k;
@@ -110,6 +116,14 @@ static int rebuild_lists(Server& sv) {
     sv.walls_cap = std::max(walls.size(), sv.walls_cap * 2);
     SV_TRY(hipMalloc((void**)&sv.walls_dev, sv.walls_cap * sizeof(WallRef)));
   }
+  if (ptiles.size() > sv.ptiles_cap) {
+    if (sv.ptiles_dev) SV_TRY(hipFree(sv.ptiles_dev));
+    sv.ptiles_cap = std::max(ptiles.size(), sv.ptiles_cap * 2);
+    SV_TRY(hipMalloc((void**)&sv.ptiles_dev, sv.ptiles_cap * sizeof(TileRef)));
+  }
+  if (!ptiles.empty())
+    SV_TRY(hipMemcpyAsync(sv.ptiles_dev, ptiles.data(), ptiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
+                          sv.stream));
   if (!tiles.empty())
     SV_TRY(hipMemcpyAsync(sv.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
                           sv.stream));
@@ -119,6 +133,7 @@ static int rebuild_lists(Server& sv) {
   SV_TRY(hipStreamSynchronize(sv.stream));
   sv.ntiles = (int)tiles.size();
   sv.nwalls = (int)walls.size();
+  sv.nptiles = (int)ptiles.size();
   sv.lists_dirty = false;
   return LBM_OK;
 }
@@ -206,7 +221,7 @@ extern "C" int lbm_server_create(int32_t precision, int32_t rk_method, size_t hb
   if (!out || max_slots < 1 || hbm_budget_bytes == 0) return fail_with(LBM_ERR_ARG, "bad server arguments");
   if (precision != LBM_PREC_FP64 && precision != LBM_PREC_MIXED)
     return fail_with(LBM_ERR_ARG, "unknown precision %d", precision);
-  if (rk_method != 1) return fail_with(LBM_ERR_ARG, "RKMETHOD %d not supported by the server", rk_method);
+  if (rk_method < 0 || rk_method > 3) return fail_with(LBM_ERR_ARG, "RKMETHOD %d unknown", rk_method);
   if (lbm_device_count() < 1) return fail_with(LBM_ERR_NOGPU, "no gfx950 device visible");
   std::unique_ptr<Server> sv(new Server());
   sv->precision = precision;
@@ -224,6 +239,8 @@ extern "C" int lbm_server_submit(lbm_server* h, const lbm_params* params, int32_
   if (!h || !params || !bnd || !props || nx < 4 || ny < 4 || nsteps < 0 || props->num_props < 1)
     return fail_with(LBM_ERR_ARG, "bad request");
   Server& sv = *reinterpret_cast<Server*>(h);
+  if (params->rk_method != sv.rk_method)
+    return fail_with(LBM_ERR_ARG, "request RKMETHOD %d differs from the server's %d", params->rk_method, sv.rk_method);
   PendingReq r;
   r.params = *params;
   r.nx = nx;
@@ -265,7 +282,8 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     std::vector<Solver*> members;
     members.reserve(sv.active.size());
     for (auto& a : sv.active) members.push_back(a.solver);
-    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls};
+    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls, sv.ptiles_dev, sv.nptiles,
+                 sv.rk_method};
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
     st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round);
