@@ -122,3 +122,11 @@ def test_case_builders_are_consistent():
     r = cases.riemann(16)
     assert np.array_equal(r.bnd, np.rot90(np.rot90(r.bnd)) * 0 + r.bnd)
     assert r.bnd[12, 12] == 0 and r.bnd[2, 12] == 1 and r.bnd[2, 2] == 2 and r.bnd[12, 2] == 3
+
+
+def test_cpp_dropin_header_compiles(tmp_path):
+    """include/gpuLBM.hpp compiles as plain C++ against the C ABI."""
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "gpuLBM.hpp"\nint main(){ gpuLBM l; l.Nx = 4; (void)l; return 0; }\n')
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                   check=True)

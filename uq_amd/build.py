@@ -55,8 +55,16 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ drop-in driver (host code only, linked against the library)
+    drv = os.path.join(OUT_DIR, "lbm_run")
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(HERE, "host", "lbm_run.cpp"), "-o", drv, f"-L{OUT_DIR}", "-luqlbm",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {drv}")
     return LIB
 
 
