@@ -68,3 +68,17 @@ def test_snapshot_round_trip(tmp_path):
         np.testing.assert_array_equal(back[k], macro[k])
     np.testing.assert_array_equal(back["f"], f)
     assert back["t"] == 0.5 and back["dy"] == 0.2
+
+
+def test_reference_deck_passes_geometry_validation():
+    """The library's host-side check of its stencil addressing accepts the
+    full 1001 x 101 reference deck (status OK with a GPU, NOGPU without)."""
+    import ctypes as C
+    from uq_amd import capi, lbm
+    c = deck.read_mat_deck(JACOBS)
+    d = capi.PackedDeck(c.params, c.bnd, c.props)
+    h = C.c_void_p()
+    st = lbm.lib().lbm_create(C.byref(d.params), c.nx, c.ny, d.bnd_ptr(), C.byref(d.props), 0, C.byref(h))
+    if st == 0:
+        lbm.lib().lbm_destroy(h)
+    assert st in (capi.LBM_OK, capi.LBM_ERR_NOGPU), lbm.lib().lbm_last_error().decode()
