@@ -1,0 +1,110 @@
+"""Write the oracle fixtures under tests/golden/ (run from the repository
+root: ``python tests/golden/make_golden.py``; about two minutes on one core).
+
+Each fixture holds the macroscopic fields (rho, ux, uy, T; [x][y], FP64) of
+the CPU oracle after a run of a named deck, the deck's builder call (JSON),
+the step count, and the oracle's own sensitivity to a one-ulp perturbation
+of the initial state over the same run. The GPU tests compare the HIP path
+against these at step counts the live oracle would take too long for inside
+a test; the CPU tests re-run the short ones to show the oracle still
+produces them bit for bit.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from uq_amd import cases  # noqa: E402
+
+# name -> (builder, args, kwargs, steps, x_invariant)
+#
+# Step counts: a fully two-dimensional deck amplifies rounding quickly (the
+# plate's one-ulp sensitivity grows from 6e-14 at 40 steps to 4e-8 at 150),
+# so its fixture stops where the comparison still means something. Decks
+# that are invariant along x (Couette, shear wave) stay invariant under any
+# per-cell arithmetic that is the same in every column, on the CPU and on
+# the GPU alike; their sensitivity is measured with a perturbation that is
+# itself x-invariant, and they run to their full length.
+FIXTURES = {
+    # the 64 x 64 Couette deck at its full 2000 steps (WENO5, IMEX RK3)
+    "couette64_rk3_weno5": ("couette", [64], {}, 2000, True),
+    # Mach-2 flat plate, NND, inflow column + free-stream top row + wall
+    "plate128x64_rk3_nnd": ("flat_plate", [128, 64], {"plate_start": 16}, 60, False),
+    # four-quadrant Riemann problem, WENO5, clamp edges
+    "riemann96_rk3_weno5": ("riemann", [96], {}, 100, False),
+    # one-stage IMEX stepper (RKMETHOD 0) on a doubly periodic shear wave
+    "shear48_rk1_weno5": ("shear_wave", [48], {"rk_method": 0}, 200, True),
+}
+FIELDS = ("rho", "ux", "uy", "T")
+
+
+def build_case(spec: str) -> "cases.Case":
+    d = json.loads(spec)
+    return getattr(cases, d["fn"])(*d["args"], **d["kwargs"])
+
+
+def spec_of(name: str) -> str:
+    fn, args, kwargs = FIXTURES[name][:3]
+    return json.dumps({"fn": fn, "args": args, "kwargs": kwargs}, sort_keys=True)
+
+
+def run_oracle(case, steps: int, perturb: bool = False, x_invariant: bool = False):
+    """Run the oracle; with `perturb`, from the initial state scaled by
+    (1 +- eps) per value (random signs, the same in every column when
+    `x_invariant`)."""
+    from pyoracle import Oracle
+    o = Oracle(case)
+    if perturb:
+        f, h = o.get_dist()
+        rng = np.random.default_rng(1)
+        eps = np.finfo(np.float64).eps
+        shape = (1,) + f.shape[1:] if x_invariant else f.shape
+        o.set_dist(f * (1 + eps * rng.choice([-1.0, 1.0], shape)),
+                   h * (1 + eps * rng.choice([-1.0, 1.0], shape)))
+    st = o.step(steps)
+    m = o.get_macro()
+    o.close()
+    return st, m
+
+
+def rel_l2(a, b) -> float:
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def field_errors(m, ref) -> dict:
+    """Relative L2 of rho, T and of the velocity vector (ux, uy together:
+    uy alone is zero up to rounding in the x-invariant decks)."""
+    u = lambda d: np.stack([d["ux"], d["uy"]])
+    return {"rho": rel_l2(m["rho"], ref["rho"]), "T": rel_l2(m["T"], ref["T"]), "u": rel_l2(u(m), u(ref))}
+
+
+def make(name: str) -> str:
+    spec = spec_of(name)
+    steps, x_inv = FIXTURES[name][3:5]
+    case = build_case(spec)
+    t = time.time()
+    st, m = run_oracle(case, steps)
+    if st != 0:
+        raise RuntimeError(f"{name}: oracle failed")
+    _, mp = run_oracle(case, steps, perturb=True, x_invariant=x_inv)
+    floor = max(field_errors(mp, m).values())
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, spec=np.array(spec), steps=np.int64(steps), floor=np.float64(floor),
+                        **{k: np.ascontiguousarray(m[k], dtype=np.float64) for k in FIELDS})
+    print(f"{name}: {case.nx}x{case.ny} {steps} steps, 1-ulp sensitivity {floor:.2e}, "
+          f"{time.time() - t:.1f}s -> {os.path.relpath(path, ROOT)}")
+    return path
+
+
+if __name__ == "__main__":
+    for n in (sys.argv[1:] or FIXTURES):
+        make(n)

@@ -1,0 +1,43 @@
+"""The committed oracle fixtures under tests/golden/ are what the oracle
+produces now (CPU only; the short fixtures are re-run bit for bit, the long
+Couette run is checked for its shape and symmetry)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLDEN)
+import make_golden  # noqa: E402
+
+pytest.importorskip("pyoracle")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("name", ["plate128x64_rk3_nnd", "shear48_rk1_weno5", "riemann96_rk3_weno5"])
+def test_short_fixture_reproduces(name):
+    z = _load(name)
+    case = make_golden.build_case(str(z["spec"]))
+    st, m = make_golden.run_oracle(case, int(z["steps"]))
+    assert st == 0
+    for k in make_golden.FIELDS:
+        np.testing.assert_array_equal(m[k], z[k], err_msg=k)
+
+
+def test_couette_fixture_is_x_invariant_and_sheared():
+    z = _load("couette64_rk3_weno5")
+    assert int(z["steps"]) == 2000
+    ux = z["ux"]
+    assert ux.shape == (64, 64)
+    # periodic in x with x-invariant walls: every column is the same
+    assert np.all(ux == ux[:1, :])
+    # wall-normal velocity: small (the compressible transient), not zero
+    assert np.max(np.abs(z["uy"])) < 0.01 * 0.1
+    # fluid velocity rises from the resting wall (y = 0) to the moving one
+    prof = ux[0, 1:-1]
+    assert np.all(np.diff(prof) > -0.01) and prof[-1] - prof[0] > 0.05
+    assert 0 < float(z["floor"]) < 1e-11
