@@ -82,6 +82,8 @@ def test_create_rejects_thin_solid_feature():
     (lambda c: c.params.__setitem__("rk_method", 7), capi.LBM_ERR_ARG),
     (lambda c: c.params.__setitem__("flux_method", 3), capi.LBM_ERR_ARG),
     (lambda c: c.bnd.__setitem__((0, 0), 99), capi.LBM_ERR_ARG),
+    (lambda c: c.params.__setitem__("mu", 0.0), capi.LBM_ERR_ARG),
+    (lambda c: c.params.__setitem__("Pr", -0.7), capi.LBM_ERR_ARG),
 ])
 def test_create_rejects_bad_decks(mutate, expected):
     c = cases.couette(16)

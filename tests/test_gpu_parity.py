@@ -27,6 +27,7 @@ def _oracle_run(case, steps):
     st = o.step(steps)
     f, h = o.get_dist()
     m = o.get_macro()
+    m["step"] = int(o.consts()["step"])
     o.close()
     return st, f, h, m
 
@@ -59,7 +60,10 @@ def _gpu_run(lbm, case, steps, precision="fp64"):
     return st, f, h, m, info
 
 
-def _compare(case, steps, gpu_lib, tol=TOL64, conditioned=False):
+def _compare(case, steps, gpu_lib, tol=TOL64, conditioned=False, may_fail=False):
+    """Oracle vs GPU after `steps` steps. With `may_fail`, a deck on which
+    the oracle reports failure (the equilibrium check) must fail on the GPU
+    at the same step (the states are then blowing up and not compared)."""
     print(f"[parity] {case.name} steps={steps}", flush=True)
     if conditioned:
         floor = _oracle_floor(case, steps)
@@ -67,8 +71,14 @@ def _compare(case, steps, gpu_lib, tol=TOL64, conditioned=False):
         print(f"[parity] {case.name}: oracle 1-ulp sensitivity {floor:.2e} -> bar {tol:.2e}", flush=True)
     so, fo, ho, mo = _oracle_run(case, steps)
     sg, fg, hg, mg, info = _gpu_run(gpu_lib, case, steps)
-    assert so == 0 and sg == 0
-    assert info["step"] == steps
+    if may_fail and so != 0:
+        print(f"[parity] {case.name}: oracle failed at step {mo['step']}, GPU status {sg} at {info['step']}",
+              flush=True)
+        assert sg == so and info["step"] == mo["step"]
+        return None
+    else:
+        assert so == 0 and sg == 0
+        assert info["step"] == steps
     errs = {
         "f": rel_l2(fg, fo),
         "h": rel_l2(hg, ho),
@@ -138,5 +148,10 @@ def test_one_stage_steppers(gpu_lib, rk):
     """RKMETHOD 0/2/3 (reference RK1, EULER, EULER_EQ): Couette (walls),
     a mirrored flat plate and a periodic shear wave."""
     for c in (cases.couette(40, rk_method=rk), cases.shear_wave(32, rk_method=rk),
-              cases.flat_plate(nx=48, ny=32, plate_start=12, steps=10, rk_method=rk)):
+              cases.flat_plate(nx=48, ny=32, plate_start=12, steps=10, rk_method=rk, re_x=1e3)):
         _compare(c, 10, gpu_lib)
+    if rk == 2:
+        # explicit relaxation with dt/tau >> 1 (the Re 1e5 plate) is unstable:
+        # the equilibrium check trips on both sides at the same step
+        c = cases.flat_plate(nx=48, ny=32, plate_start=12, steps=10, rk_method=rk)
+        _compare(c, 10, gpu_lib, may_fail=True)

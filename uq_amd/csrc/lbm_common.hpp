@@ -50,6 +50,10 @@ struct InstDev {
   double sRTc;         // sqrt(R*Tc) as the equilibrium computes it
   double Tc, R, b, K, bR, RTc;
   double mu, Pr;
+  // reciprocals and differences formed once on the host so that the
+  // per-cell arithmetic multiplies instead of divides
+  double isRTc, iTc, ibR;  // 1/sqrt(R*Tc), 1/Tc, 1/(b*R)
+  double imu, Pr_m1;       // 1/mu, Pr - 1
 };
 
 // One tile of one instance: the unit of work of a block.

@@ -45,27 +45,28 @@ __host__ __device__ constexpr int eq_sb(int i) { return (i == 7 || i == 8) ? -1 
 
 // ---------------------------------------------------------------------------
 // Equilibrium polynomials: the closed forms of the reference
-// (inlineFunctions.cpp:10-61), evaluated with products instead of pow().
+// (inlineFunctions.cpp:10-61), evaluated with products instead of pow() and
+// with the constant denominators applied as reciprocal products.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T poly_f0(T rho, T u, T v, T t) {
   const T u2 = u * u, v2 = v * v;
-  return (rho / T(4)) * (T(4) + T(10) * (t * t) + u2 * u2 - T(5) * v2 + v2 * v2 +
+  return (rho * T(0.25)) * (T(4) + T(10) * (t * t) + u2 * u2 - T(5) * v2 + v2 * v2 +
                          T(10) * t * (T(-1) + u2 + v2) + u2 * (T(-5) + T(4) * v2));
 }
 template <typename T>
 __device__ __forceinline__ T poly_f1(T rho, T a, T b, T t) {
   const T a2 = a * a, b2 = b * b;
-  return (rho / T(6)) * (T(-6) * (t * t) - a * (T(1) + a) * (T(-4) + a2 + T(3) * b2) -
+  return (rho * T(1.0 / 6.0)) * (T(-6) * (t * t) - a * (T(1) + a) * (T(-4) + a2 + T(3) * b2) -
                          t * (T(-4) + T(6) * a + T(9) * a2 + T(3) * b2));
 }
 template <typename T>
 __device__ __forceinline__ T poly_f5(T rho, T a, T b, T t) {
-  return (rho / T(4)) * ((t + a + a * a) * (t + b + b * b));
+  return (rho * T(0.25)) * ((t + a + a * a) * (t + b + b * b));
 }
 template <typename T>
 __device__ __forceinline__ T poly_f9(T rho, T a, T t) {
-  return (rho / T(24)) * (T(3) * (t * t) + (T(-1) + a) * a * (T(1) + a) * (T(2) + a) +
+  return (rho * T(1.0 / 24.0)) * (T(3) * (t * t) + (T(-1) + a) * a * (T(1) + a) * (T(2) + a) +
                           t * (T(-1) + T(6) * a * (T(1) + a)));
 }
 template <typename T>
@@ -76,8 +77,8 @@ __device__ __forceinline__ T poly_h0(T K, T t, T u, T v) {
               (T(8) + T(4) * K - T(40) * u2 - T(5) * K * u2 + T(20) * u4 + K * u4 +
                (T(-5) * (T(8) + K) + T(4) * (T(15) + K) * u2) * v2 + (T(20) + K) * v4) +
           T(30) * t2 * (T(-4) + T(9) * u2 + T(9) * v2 + K * (T(-1) + u2 + v2)) +
-          T(3) * (u2 + v2) * (T(4) + u4 - T(5) * v2 + v4 + u2 * (T(-5) + T(4) * v2))) /
-         T(24);
+          T(3) * (u2 + v2) * (T(4) + u4 - T(5) * v2 + v4 + u2 * (T(-5) + T(4) * v2))) *
+         T(1.0 / 24.0);
 }
 template <typename T>
 __device__ __forceinline__ T poly_h1(T K, T t, T a, T b) {
@@ -86,8 +87,8 @@ __device__ __forceinline__ T poly_h1(T K, T t, T a, T b) {
           t * (a * (T(-4) * (T(4) + K) - T(4) * (T(7) + K) * a + (T(14) + K) * a2 + (T(19) + K) * a3) +
                (T(-4) + T(3) * a * (T(10) + K + (T(14) + K) * a)) * b2 + T(3) * (b2 * b2)) -
           t2 * (T(-16) + T(6) * a * (T(6) + T(13) * a) + T(30) * b2 +
-                K * (T(-4) + T(6) * a + T(9) * a2 + T(3) * b2))) /
-         T(12);
+                K * (T(-4) + T(6) * a + T(9) * a2 + T(3) * b2))) *
+         T(1.0 / 12.0);
 }
 template <typename T>
 __device__ __forceinline__ T poly_h5(T K, T t, T a, T b) {
@@ -96,8 +97,8 @@ __device__ __forceinline__ T poly_h5(T K, T t, T a, T b) {
           T(3) * t2 * ((T(6) + K) * a + (T(9) + K) * a2 + b * (T(6) + K + (T(9) + K) * b)) +
           T(3) * t *
               (a2 * a + a2 * a2 + b2 * b * (T(1) + b) + a * b * (T(6) + K + (T(9) + K) * b) +
-               a2 * b * (T(9) + K + (T(12) + K) * b))) /
-         T(24);
+               a2 * b * (T(9) + K + (T(12) + K) * b))) *
+         T(1.0 / 24.0);
 }
 template <typename T>
 __device__ __forceinline__ T poly_h9(T K, T t, T a, T b) {
@@ -105,8 +106,8 @@ __device__ __forceinline__ T poly_h9(T K, T t, T a, T b) {
   return ((T(16) + T(3) * K) * t3 +
           t * a * (T(-8) + K * (T(-1) + a) * (T(1) + a) * (T(2) + a) + a * (T(-7) + T(2) * a * (T(11) + T(8) * a))) +
           t * (T(-1) + T(6) * a * (T(1) + a)) * b2 + (T(-1) + a) * a * (T(1) + a) * (T(2) + a) * (a2 + b2) +
-          t2 * (T(-4) + T(36) * a + T(51) * a2 + K * (T(-1) + T(6) * a * (T(1) + a)) + T(3) * b2)) /
-         T(48);
+          t2 * (T(-4) + T(36) * a + T(51) * a2 + K * (T(-1) + T(6) * a * (T(1) + a)) + T(3) * b2)) *
+         T(1.0 / 48.0);
 }
 
 // Normalised equilibrium inputs of one cell.
@@ -141,31 +142,36 @@ __device__ __forceinline__ void eq_dir(const EqIn<T>& q, T& fe, T& he) {
 }
 
 // Moments of one cell (reference macroProp, cpuLBM.cpp:916-970), FP64.
+// The relaxation times enter every update as divisors; the cell keeps their
+// reciprocals: 1/tf = p/mu, 1/th = Pr/tf, and since thf = th tf/(tf - th),
+// 1/thf = 1/th - 1/tf = (Pr - 1)/tf (one rounding; Pr - 1 is exact for
+// 0.5 <= Pr <= 2).
 struct Macro {
-  double rho, ux, uy, T, p, tf, th, thf;
+  double rho, ux, uy, T, p;
+  double itf, ith, ithf;  // 1/tf, 1/th, 1/thf
 };
 
 __device__ __forceinline__ void finish_macro(const InstDev& in, double rho, double rux, double ruy,
                                              double sh, Macro& m) {
+  const double irho = 1.0 / rho;
   m.rho = rho;
-  m.ux = rux / rho;
-  m.uy = ruy / rho;
-  const double ax = fabs(m.ux), ay = fabs(m.uy);
-  const double usq = ax * ax + ay * ay;
-  m.T = 2.0 * (sh / rho - usq / 2.0) / in.bR;
+  m.ux = rux * irho;
+  m.uy = ruy * irho;
+  const double usq = m.ux * m.ux + m.uy * m.uy;
+  m.T = (2.0 * (sh * irho) - usq) * in.ibR;
   m.p = rho * in.R * m.T;
-  m.tf = in.mu / m.p;
-  m.th = m.tf / in.Pr;
-  m.thf = (m.th * m.tf) / (m.tf - m.th);
+  m.itf = m.p * in.imu;
+  m.ith = m.itf * in.Pr;
+  m.ithf = m.itf * in.Pr_m1;
 }
 
 template <typename T>
 __device__ __forceinline__ EqIn<T> eq_inputs(const InstDev& in, const Macro& m) {
   EqIn<T> q;
   q.rho = T(m.rho);
-  q.u = T(m.ux / in.sRTc);
-  q.v = T(m.uy / in.sRTc);
-  q.t = T(m.T / in.Tc);
+  q.u = T(m.ux * in.isRTc);
+  q.v = T(m.uy * in.isRTc);
+  q.t = T(m.T * in.iTc);
   q.K = T(in.K);
   q.scale = T(m.rho * in.RTc);
   return q;

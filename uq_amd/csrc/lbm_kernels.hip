@@ -140,12 +140,14 @@ __device__ __forceinline__ void moments26(const InstDev& in, const double* f, co
   finish_macro(in, rho, rux, ruy, sh, m);
 }
 
-// Stage-1/2 coefficients of one cell.
+// Stage-1/2 coefficients of one cell. Every divisor of the stage updates is
+// applied as a reciprocal formed once per cell.
 template <typename TC>
 struct Rec12 {
   EqIn<TC> q;
-  TC cU, cV;          // c*U, c*V: e_i . u = lat_x(i)*cU + lat_y(i)*cV
-  TC af, ah, ahf, g;  // dt/(2 tf), dt/(2 th), dt/(2 thf), 1/thf
+  TC cU, cV;           // c*U, c*V: e_i . u = lat_x(i)*cU + lat_y(i)*cV
+  TC af, ah, ahf, g;   // dt/(2 tf), dt/(2 th), dt/(2 thf), 1/thf
+  TC iaf, iah;         // 1/(1 + af), 1/(1 + ah)
 };
 
 template <typename TC>
@@ -154,11 +156,13 @@ __device__ __forceinline__ Rec12<TC> make_rec12(const InstDev& in, const Macro& 
   r.q = eq_inputs<TC>(in, m);
   r.cU = TC(in.c) * TC(m.ux);
   r.cV = TC(in.c) * TC(m.uy);
-  const TC dt = TC(in.dt);
-  r.af = dt / (TC(2) * TC(m.tf));
-  r.ah = dt / (TC(2) * TC(m.th));
-  r.ahf = dt / (TC(2) * TC(m.thf));
-  r.g = TC(1) / TC(m.thf);
+  const TC hdt = TC(in.dt) * TC(0.5);
+  r.af = hdt * TC(m.itf);
+  r.ah = hdt * TC(m.ith);
+  r.ahf = hdt * TC(m.ithf);
+  r.g = TC(m.ithf);
+  r.iaf = TC(1) / (TC(1) + r.af);
+  r.iah = TC(1) / (TC(1) + r.ah);
   return r;
 }
 
@@ -167,12 +171,11 @@ __device__ __forceinline__ Rec12<TC> make_rec12(const InstDev& in, const Macro& 
 template <int I, typename TC>
 __device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC& fe, TC& he, TC& f2, TC& h2) {
   eq_dir<I>(r.q, fe, he);
-  const TC f1 = (fv + r.af * fe) / (TC(1) + r.af);
+  const TC f1 = (fv + r.af * fe) * r.iaf;
   const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
-  const TC h1 = (hv - r.ahf * eu * (fe - f1) + r.ah * he) / (TC(1) + r.ah);
-  f2 = (fv - r.af * (fe - f1) + r.af * fe) / (TC(1) + r.af);
-  h2 = (hv - hdt * ((eu * r.g) * (fe - f2) - (eu * r.g) * (fe - f1)) - r.ah * (he - h1) + r.ah * he) /
-       (TC(1) + r.ah);
+  const TC h1 = (hv - r.ahf * eu * (fe - f1) + r.ah * he) * r.iah;
+  f2 = (fv - r.af * (fe - f1) + r.af * fe) * r.iaf;
+  h2 = (hv - hdt * ((eu * r.g) * (fe - f2) - (eu * r.g) * (fe - f1)) - r.ah * (he - h1) + r.ah * he) * r.iah;
 }
 
 // Own cell of a pointwise launch: 64 x 4 row tiles, so each wavefront reads
@@ -278,12 +281,10 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
   finish_macro(in, rho, rux, ruy, sh, m1);
   finish_macro(in, rho - drho, rux - drux, ruy - druy, sh - dsh, m3);
   const Rec12<TC> r = make_rec12<TC>(in, m1);
-  const TC dt = TC(in.dt), hdt = dt / TC(2);
-  const TC tf2 = TC(m1.tf), th2 = TC(m1.th), thf2 = TC(m1.thf);
-  const TC af2 = dt / (TC(2) * tf2), ah2 = dt / (TC(2) * th2);
+  const TC hdt = TC(in.dt) * TC(0.5);
   const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
-  const TC tf3 = TC(m3.tf), th3 = TC(m3.th), thf3 = TC(m3.thf);
-  const TC af3 = dt / (TC(2) * tf3), ah3 = dt / (TC(2) * th3);
+  const TC af3 = hdt * TC(m3.itf), ah3 = hdt * TC(m3.ith), ithf3 = TC(m3.ithf);
+  const TC iaf3 = TC(1) / (TC(1) + af3), iah3 = TC(1) / (TC(1) + ah3);
   const TC cU3 = TC(in.c) * TC(m3.ux), cV3 = TC(in.c) * TC(m3.uy);
   double sumfe = 0, sumabs = 0;
   auto one = [&](auto Ic) {
@@ -298,19 +299,19 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     s12<I>(r, hdt, fv, hv, fe, he, f2, h2);
     const TC D = fe - f2;
     const TC eu2 = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
-    const TC A2 = (eu2 / thf2) * D;
-    const TC Pf = (fv - dF) + af2 * D;
-    const TC Qh = ((hv - dH) - hdt * A2) + ah2 * (he - h2);
+    const TC A2 = (eu2 * r.g) * D;
+    const TC Pf = (fv - dF) + r.af * D;
+    const TC Qh = ((hv - dH) - hdt * A2) + r.ah * (he - h2);
     TC fe3, he3;
     eq_dir<I>(q3, fe3, he3);
     sumfe += double(fe3);
     sumabs += fabs(double(fe3));
-    const TC f3 = (Pf + af3 * fe3) / (TC(1) + af3);
+    const TC f3 = (Pf + af3 * fe3) * iaf3;
     const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
-    const TC A3 = (eu3 / thf3) * (fe3 - f3);
-    const TC h3 = ((Qh - hdt * A3) + ah3 * he3) / (TC(1) + ah3);
-    const TC gf = (Pf + TC(0.5) * dF) + hdt * ((fe3 - f3) / tf3);
-    const TC gh = ((Qh + TC(0.5) * dH) + hdt * ((he3 - h3) / th3)) - hdt * A3;
+    const TC A3 = (eu3 * ithf3) * (fe3 - f3);
+    const TC h3 = ((Qh - hdt * A3) + ah3 * he3) * iah3;
+    const TC gf = (Pf + TC(0.5) * dF) + af3 * (fe3 - f3);
+    const TC gh = ((Qh + TC(0.5) * dH) + ah3 * (he3 - h3)) - hdt * A3;
     f3b[P * I + o] = TS(f3);
     f3b[P * (kQ + I) + o] = TS(h3);
     out[P * I + o] = TS(gf);
@@ -320,9 +321,13 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
   flag_failure(in, step, m3.rho, sumfe, eq_check_tol<TC>(sumabs));
 }
 
+// Two waves per SIMD: the FP64 instantiation otherwise takes 262 registers
+// (one wave per SIMD); capping it at 256 costs a few spills and hides twice
+// the memory latency (4096^2 FP64: 5.9 -> 4.2 ms per launch).
 template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) void k_s3(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                            int step) {
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k_s3(const InstDev* insts,
+                                                                                   const TileRef* tiles,
+                                                                                   int ntiles, int par, int step) {
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -358,6 +363,7 @@ struct Rec1 {
   EqIn<TC> q;
   TC cU, cV;
   TC bf, bh, dt, ithf;  // dt/tf, dt/th, dt, 1/thf
+  TC ibf, ibh;          // 1/(1 + bf), 1/(1 + bh)
 };
 
 template <typename TC>
@@ -367,9 +373,11 @@ __device__ __forceinline__ Rec1<TC> make_rec1(const InstDev& in, const Macro& m)
   r.cU = TC(in.c) * TC(m.ux);
   r.cV = TC(in.c) * TC(m.uy);
   r.dt = TC(in.dt);
-  r.bf = r.dt / TC(m.tf);
-  r.bh = r.dt / TC(m.th);
-  r.ithf = TC(1) / TC(m.thf);
+  r.bf = r.dt * TC(m.itf);
+  r.bh = r.dt * TC(m.ith);
+  r.ithf = TC(m.ithf);
+  r.ibf = TC(1) / (TC(1) + r.bf);
+  r.ibh = TC(1) / (TC(1) + r.bh);
   return r;
 }
 
@@ -377,9 +385,9 @@ __device__ __forceinline__ Rec1<TC> make_rec1(const InstDev& in, const Macro& m)
 template <int I, typename TC>
 __device__ __forceinline__ void s1(const Rec1<TC>& r, TC fv, TC hv, TC& fe, TC& he, TC& f1, TC& h1) {
   eq_dir<I>(r.q, fe, he);
-  f1 = (fv + r.bf * fe) / (TC(1) + r.bf);
+  f1 = (fv + r.bf * fe) * r.ibf;
   const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
-  h1 = (hv - r.dt * eu * ((fe - f1) * r.ithf) + r.bh * he) / (TC(1) + r.bh);
+  h1 = (hv - r.dt * eu * ((fe - f1) * r.ithf) + r.bh * he) * r.ibh;
 }
 
 template <typename TS, typename TC, int... Is>

@@ -334,6 +334,11 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.RTc = s.p.R * s.Tc;
   d.mu = s.p.mu;
   d.Pr = s.p.Pr;
+  d.isRTc = 1.0 / d.sRTc;
+  d.iTc = 1.0 / d.Tc;
+  d.ibR = 1.0 / d.bR;
+  d.imu = 1.0 / d.mu;
+  d.Pr_m1 = d.Pr - 1.0;
 }
 
 static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls,
@@ -410,6 +415,10 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
     return fail_with(LBM_ERR_ARG, "RKMETHOD %d unknown (0 RK1, 1 RK3, 2 EULER, 3 EULER_EQ)", params->rk_method);
   if (params->flux_method != 0 && params->flux_method != 1)
     return fail_with(LBM_ERR_ARG, "FMETHOD %d unknown", params->flux_method);
+  // the relaxation times tf = mu/p and th = tf/Pr need a positive viscosity
+  // and Prandtl number (the reference would divide by zero further on)
+  if (!(params->mu > 0) || !(params->Pr > 0) || !std::isfinite(params->mu) || !std::isfinite(params->Pr))
+    return fail_with(LBM_ERR_ARG, "mu = %g and Pr = %g must be positive", params->mu, params->Pr);
   if (props->num_props < 1 || !props->density || !props->pressure || !props->temp || !props->velocity_x ||
       !props->velocity_y || !props->cell_type)
     return fail_with(LBM_ERR_ARG, "incomplete property table");
