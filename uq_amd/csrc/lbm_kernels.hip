@@ -648,31 +648,48 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
   void* src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
   TS* dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
   const TC rdx = TC(in.rdx), rdy = TC(in.rdy);
+  // Staging slots of this thread, fixed for the tile: slot r < kOwn is an
+  // own cell, then the x halo bands (kNXB), then the y halo bands (kNYB).
+  // Each slot keeps its LDS position, grid offset and meta byte; a direction
+  // stages the own slots and the bands along its nonzero axes.
+  constexpr int kSlots = (kOwn + kNXB + kNYB + kNT - 1) / kNT;
+  int s_lds[kSlots], s_off[kSlots], s_meta[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; j++) {
+    const int r = t + j * kNT;
+    int lx = 0, ly = 0, kind = 3;  // 0 own, 1 x band, 2 y band, 3 none
+    if (r < kOwn) {
+      lx = r % kTX;
+      ly = r / kTX;
+      kind = 0;
+    } else if (r < kOwn + kNXB) {
+      const int k = r - kOwn, side = k / (kH * kTY), kk = k % (kH * kTY);
+      ly = kk / kH;
+      lx = side ? kTX + kk % kH : -kH + kk % kH;
+      kind = 1;
+    } else if (r < kOwn + kNXB + kNYB) {
+      const int k = r - kOwn - kNXB, side = k / (kH * kTX), kk = k % (kH * kTX);
+      lx = kk % kTX;
+      ly = side ? kTY + kk / kTX : -kH + kk / kTX;
+      kind = 2;
+    }
+    s_lds[j] = (ly + kH) * kRX + (lx + kH);
+    s_off[j] = kind < 3 ? R.off[ly + kH][lx + kH] : 0;
+    s_meta[j] = kind < 3 ? (int)R.meta[ly + kH][lx + kH] | (kind << 8) : (3 << 8);
+  }
   for (int I = 1; I < kQ; I++) {
     const int ex = c_lat_x[I], ey = c_lat_y[I];
     const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
     const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
-    // staging: own rect, then the halo bands this direction needs
-    const int nxb = ex ? kNXB : 0, nyb = ey ? kNYB : 0;
-    for (int r = t; r < kOwn + nxb + nyb; r += kNT) {
-      int lx, ly;
-      if (r < kOwn) {
-        lx = r % kTX;
-        ly = r / kTX;
-      } else if (r < kOwn + nxb) {
-        const int k = r - kOwn, side = k / (kH * kTY), kk = k % (kH * kTY);
-        ly = kk / kH;
-        lx = side ? kTX + kk % kH : -kH + kk % kH;
-      } else {
-        const int k = r - kOwn - nxb, side = k / (kH * kTX), kk = k % (kH * kTX);
-        lx = kk % kTX;
-        ly = side ? kTY + kk / kTX : -kH + kk / kTX;
-      }
-      const unsigned m = R.meta[ly + kH][lx + kH];
+#pragma unroll
+    for (int j = 0; j < kSlots; j++) {
+      const int kind = s_meta[j] >> 8;
+      if (kind == 3 || (kind == 1 && !ex) || (kind == 2 && !ey)) continue;
+      const unsigned m = (unsigned)s_meta[j];
       const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
-      const long long o = R.off[ly + kH][lx + kH];
-      S.f[ly + kH][lx + kH] = TC(plane<TS>(src, P, d)[o]);
-      S.h[ly + kH][lx + kH] = TC(plane<TS>(src, P, kQ + d)[o]);
+      const long long o = s_off[j];
+      (&S.f[0][0])[s_lds[j]] = TC(plane<TS>(src, P, d)[o]);
+      (&S.h[0][0])[s_lds[j]] = TC(plane<TS>(src, P, kQ + d)[o]);
     }
     __syncthreads();
     // faces: thread t owns x face (row ty, owner column tx - [sx > 0]) and
