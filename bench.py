@@ -2,18 +2,23 @@
 """Benchmark: MLUPS of the D2Q13 thermal FD-LBM (f+h, WENO5, RKMETHOD 1) on
 one MI355X.
 
-Headline (`value`): aggregate MLUPS of the batched, mixed-precision UQ
-ensemble — 256 Couette / periodic shear-wave instances, square sides drawn
-from {64,128,256,512}, sampled wall speed, Mach, Pr and tau, all advanced
-together with one launch per RK stage. A step is one RK step of every
-instance; MLUPS = sum(nx*ny) * steps / device time / 1e6, timed with HIP
-events around the step loop, state resident in HBM.
+Headline (`value`): aggregate MLUPS of the batched FP64 UQ ensemble — 256
+Couette / periodic shear-wave instances, square sides drawn from
+{64,128,256,512}, sampled wall speed, Mach, Pr and tau (seed 2026), all
+advanced together with one launch per RK stage. A step is one RK step of
+every instance; MLUPS = sum(nx*ny) * steps / seconds / 1e6, with the
+seconds taken by the host clock around exactly `--steps` steps (the batch
+call synchronises the stream before returning), state resident in HBM.
+FP64 is the headline precision because it is the one that meets the
+relative-L2 <= 1e-6 bar of the metric (it matches the CPU oracle to
+<= 1e-12); the mixed-precision path is reported beside it with its measured
+error against FP64.
 
-Beside it, on the same ensemble: the plain one-instance-per-launch FP64 path
-(a step sample, per-step cost is constant), the accuracy of the mixed path
-against FP64 GPU runs (relative L2 of rho, u, T), the 4096^2 four-quadrant
-Riemann problem in FP64 with its HBM roofline fraction, the served request
-stream (p50/p99 time to solution), and the CPU oracle timed on this host.
+Beside it, on the same ensemble: the batched mixed path and its accuracy,
+the plain one-instance-per-launch FP64 path (a step sample; per-step cost is
+constant), the 4096^2 four-quadrant Riemann problem in FP64, the served
+request stream (continuous batching, p50/p99 time to solution, offered vs
+achieved load), and the CPU oracle timed on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -21,7 +26,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -35,7 +39,8 @@ from uq_amd import cases  # noqa: E402
 from uq_amd import lbm  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-STATE_VALUES_PER_CELL = 26
+STEP_KERNELS = ("s12", "flux2", "s3", "flux3")
+METRIC = "MLUPS (f+h, WENO5-RK3) plain vs batched at L2<=1e-6; serve p50/p99 latency"
 
 
 def log(msg: str) -> None:
@@ -52,16 +57,112 @@ def kernel_bytes_per_lup(es: int) -> dict:
     return {"s12": 52 * es, "flux2": 50 * es, "s3": 102 * es, "flux3": 78 * es}
 
 
-def run_ensemble(members, precision: str, steps: int, warmup: int, timing: bool):
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes committed under profiles/
+    (copy-kernel calibrated), or None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            tab = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    rec = tab.get(workload, {}).get(kernel)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def batched_leg(members, precision: str, steps: int, warmup: int, timing_steps: int):
+    """Warm up, time exactly `steps` steps of the whole batch by the host
+    clock, then take a short per-kernel timing sample (HIP events)."""
     solvers = [lbm.Solver(c, precision) for c in members]
     b = lbm.Batch(solvers)
     if warmup:
         b.step(warmup)
-    lbm.set_kernel_timing(timing)
+    t0 = time.perf_counter()
     st = b.step(steps)
-    t = lbm.last_timing()
+    sec = time.perf_counter() - t0
+    lbm.set_kernel_timing(True)
+    b.step(timing_steps)
+    kt = lbm.last_timing()["kernels"]
     lbm.set_kernel_timing(False)
-    return solvers, b, st, t
+    return solvers, b, st, sec, kt
+
+
+def roofline(ktimes: dict, cells: int, es: int, workload: str) -> dict:
+    kb = kernel_bytes_per_lup(es)
+    dom = max(STEP_KERNELS, key=lambda k: ktimes.get(k, {}).get("avg_ms", 0.0))
+    dom_ms = ktimes[dom]["avg_ms"]
+    achieved = kb[dom] * cells / (dom_ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "kernel": dom,
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "bytes_per_lup": kb[dom], "avg_launch_ms": round(dom_ms, 4),
+        "traffic": measured_traffic(dom, workload),
+    }
+
+
+def accuracy(members, steps: int, n: int) -> dict:
+    """Worst relative L2 of the mixed path against the FP64 path (rho, T and
+    the velocity vector) over `n` instances spread across the sizes."""
+    pick = sorted(range(len(members)), key=lambda k: (members[k].cells, k))
+    idx = sorted({pick[int(round(q * (len(pick) - 1)))] for q in np.linspace(0, 1, min(n, len(pick)))})
+    sub = [members[k].with_steps(steps) for k in idx]
+    mix = [lbm.Solver(c, "mixed") for c in sub]
+    ref = [lbm.Solver(c, "fp64") for c in sub]
+    lbm.Batch(mix).step(steps)
+    lbm.Batch(ref).step(steps)
+    worst = {"rho": 0.0, "u": 0.0, "T": 0.0}
+    for a, r in zip(mix, ref):
+        ma, mr = a.get_macro(), r.get_macro()
+        worst["rho"] = max(worst["rho"], lbm.rel_l2(ma["rho"], mr["rho"]))
+        worst["T"] = max(worst["T"], lbm.rel_l2(ma["T"], mr["T"]))
+        worst["u"] = max(worst["u"], lbm.rel_l2(np.stack([ma["ux"], ma["uy"]]), np.stack([mr["ux"], mr["uy"]])))
+    for s in mix + ref:
+        s.close()
+    return {"steps": steps, "instances": len(sub), **{k: float(f"{v:.3e}") for k, v in worst.items()}}
+
+
+def serve_leg(rate: float, duration: float, seed: int, tick_steps: int, max_wall: float) -> dict:
+    """Continuous batching of a Poisson request stream (sides log-uniform in
+    [64, 1024], 500..5000 steps, UQ parameters) on the FP64 server: requests
+    are submitted at their arrival times, the server advances every admitted
+    request together `tick_steps` steps per tick, and each request's time to
+    solution runs from its scheduled arrival to its retirement."""
+    reqs = cases.request_stream(rate=rate, duration=duration, seed=seed)
+    offered = sum(r.case.cells * r.case.nsteps for r in reqs)
+    srv = lbm.Server("fp64", rk_method=1, hbm_budget_bytes=200 << 30, max_slots=1024)
+    t0 = lbm.clock_s()
+    done, i = [], 0
+    while len(done) < len(reqs):
+        now = lbm.clock_s() - t0
+        while i < len(reqs) and reqs[i].arrival_s <= now:
+            srv.submit(reqs[i].case, reqs[i].tag)
+            i += 1
+        active, queued = srv.tick(tick_steps)
+        done += srv.poll()
+        if active == 0 and queued == 0 and i < len(reqs):
+            time.sleep(max(0.0, reqs[i].arrival_s - (lbm.clock_s() - t0)))
+        if lbm.clock_s() - t0 > max_wall:
+            break
+    wall = lbm.clock_s() - t0
+    arrival = {r.tag: t0 + r.arrival_s for r in reqs}
+    lat = np.array([d["finish_s"] - arrival[d["tag"]] for d in done])
+    ok = sum(1 for d in done if d["status"] == 0)
+    lups = srv.stats()["lups"]
+    srv.close()
+    out = {
+        "requests": len(reqs), "completed": len(done), "ok": ok, "arrival_window_s": duration,
+        "rate_per_s": rate, "tick_steps": tick_steps, "wall_s": round(wall, 2),
+        "offered_glups": round(offered / duration / 1e9, 3),
+        "achieved_glups": round(lups / wall / 1e9, 3),
+        "precision": "fp64",
+    }
+    if len(lat):
+        out.update(p50_s=round(float(np.percentile(lat, 50)), 3), p99_s=round(float(np.percentile(lat, 99)), 3))
+    if len(done) < len(reqs):
+        out["note"] = f"stopped after {max_wall:.0f} s with {len(reqs) - len(done)} requests unfinished"
+    return out
 
 
 def main() -> int:
@@ -71,16 +172,21 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--timing-steps", type=int, default=10, help="steps of the per-kernel timing sample")
     ap.add_argument("--plain-steps", type=int, default=20, help="step sample of the plain path")
     ap.add_argument("--riemann-n", type=int, default=4096)
     ap.add_argument("--riemann-steps", type=int, default=20)
-    ap.add_argument("--accuracy-steps", type=int, default=0,
-                    help="steps of the mixed-vs-FP64 check (0: same as --steps)")
+    ap.add_argument("--accuracy-steps", type=int, default=200, help="steps of the mixed-vs-FP64 check")
     ap.add_argument("--accuracy-instances", type=int, default=16)
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the served stream")
+    ap.add_argument("--serve-tick", type=int, default=50, help="steps per server tick")
+    ap.add_argument("--serve-max-wall", type=float, default=150.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--quick", action="store_true", help="small smoke-sized run")
-    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline leg only")
+    ap.add_argument("--legs", default="all",
+                    help="comma list of legs to run (headline,mixed,plain,riemann,serve,cpu); for profiling")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -99,138 +205,133 @@ def main() -> int:
         return 0
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
-                                   "DESIGN.md but not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU (one GPU's share of the work); the "
+                                   "multi-GPU sharding is a design, not launched", "n_gpus": args.gpus}))
         return 2
     if args.quick:
         args.instances, args.steps, args.plain_steps = 16, 50, 5
         args.riemann_n, args.riemann_steps = 512, 5
-        args.serve_seconds, args.cpu_seconds = 3.0, 3.0
+        args.serve_seconds, args.cpu_seconds, args.accuracy_steps = 0.5, 3.0, 20
     if lbm.device_count() < 1:
         raise SystemExit("no gfx950 device visible")
 
+    legs = {"headline", "mixed", "plain", "riemann", "serve", "cpu"} if args.legs == "all" else \
+        set(args.legs.split(","))
+    if args.no_extras:
+        legs = {"headline"}
     members = cases.uq_ensemble(args.instances, seed=args.seed, steps=args.steps)
     cells = sum(c.cells for c in members)
+    workload = f"uq_ensemble_{len(members)}_seed{args.seed}"
     log(f"ensemble: {len(members)} instances, {cells} cells, steps={args.steps} warmup={args.warmup}")
 
-    # --- headline: batched mixed precision --------------------------------
-    solvers, batch, st, t = run_ensemble(members, "mixed", args.steps, args.warmup, timing=True)
-    ms = t["total_ms"]
-    mlups = cells * args.steps / (ms * 1e-3) / 1e6
-    log(f"batched mixed: {ms:.1f} ms for {args.steps} steps -> {mlups:.1f} MLUPS (status {st})")
-    kb = kernel_bytes_per_lup(4)
-    ktimes = t["kernels"]
-    dom = max(("s12", "flux2", "s3", "flux3"), key=lambda k: ktimes.get(k, {}).get("avg_ms", 0.0))
-    dom_ms = ktimes[dom]["avg_ms"]
-    achieved = kb[dom] * cells / (dom_ms * 1e-3) / 1e9
-    step_bytes = 520.0  # 130 values per LUP in FP32 storage (fused two-stage model)
+    # --- headline: batched FP64 ------------------------------------------
+    if "headline" in legs:
+        solvers, batch, st, sec, kt = batched_leg(members, "fp64", args.steps, args.warmup, args.timing_steps)
+    else:  # profiling runs of other legs
+        solvers, batch, st, sec, kt = [], None, None, float("nan"), {}
+    mlups = cells * args.steps / sec / 1e6
+    log(f"batched fp64: {sec:.2f} s for {args.steps} steps -> {mlups:.1f} MLUPS (status {st})")
     result = {
-        "metric": "MLUPS (f+h, WENO5-RK3) plain vs batched at L2<=1e-6; serve p50/p99 latency",
+        "metric": METRIC,
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms / args.steps, 4),
+        "ms_per_step": round(sec * 1e3 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (bulk) + f64 (moments, WENO weights)",
-        "data": "synthetic (seeded UQ ensemble decks)",
+        "dtype": "f64",
+        "data": "synthetic (seeded UQ ensemble decks, seed %d)" % args.seed,
         "config": {
-            "workload": f"uq_ensemble_{len(members)} (batched, mixed precision, one launch per RK stage)",
+            "workload": f"uq_ensemble_{len(members)} batched FP64, one launch per RK stage for all instances",
             "instances": len(members), "cells": cells, "seed": args.seed,
             "sides": [64, 128, 256, 512], "kinds": ["couette", "shear"],
             "rk_method": 1, "flux_method": "WENO5", "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm", "kernel": dom,
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "bytes_per_lup": kb[dom], "avg_launch_ms": round(dom_ms, 4),
-            "traffic": None,
-        },
-        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in ktimes.items()},
-        "step_roofline_frac": round(mlups * step_bytes / 1e3 / HBM_PEAK_GBS, 4),
+        "roofline": roofline(kt, cells, 8, workload + "_fp64") if kt else None,
+        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kt.items()},
+        "step_roofline_frac": round(mlups * 1040.0 / 1e3 / HBM_PEAK_GBS, 4),
+        "status": st,
     }
+    if batch is not None:
+        batch.close()
+    for s in solvers:
+        s.close()
 
-    if not args.no_extras:
-        # --- accuracy of the mixed path against FP64 -------------------------
-        acc_steps = args.accuracy_steps or args.steps
-        if acc_steps == args.steps + args.warmup or True:
-            pick = sorted(range(len(members)), key=lambda k: (members[k].cells, k))
-            idx = [pick[int(round(q * (len(pick) - 1)))] for q in np.linspace(0, 1, min(args.accuracy_instances, len(pick)))]
-            idx = sorted(set(idx))
-            sub = [members[k].with_steps(acc_steps) for k in idx]
-            mix = [lbm.Solver(c, "mixed") for c in sub]
-            ref = [lbm.Solver(c, "fp64") for c in sub]
-            lbm.Batch(mix).step(acc_steps)
-            lbm.Batch(ref).step(acc_steps)
-            worst = {"rho": 0.0, "u": 0.0, "T": 0.0}
-            for a, r in zip(mix, ref):
-                ma, mr = a.get_macro(), r.get_macro()
-                worst["rho"] = max(worst["rho"], lbm.rel_l2(ma["rho"], mr["rho"]))
-                worst["T"] = max(worst["T"], lbm.rel_l2(ma["T"], mr["T"]))
-                worst["u"] = max(worst["u"], lbm.rel_l2(np.stack([ma["ux"], ma["uy"]]),
-                                                        np.stack([mr["ux"], mr["uy"]])))
-            result["mixed_vs_fp64_rel_l2"] = {"steps": acc_steps, "instances": len(sub),
-                                              **{k: float(f"{v:.3e}") for k, v in worst.items()}}
-            log(f"mixed vs fp64 after {acc_steps} steps on {len(sub)} instances: {worst}")
-            for s in mix + ref:
-                s.close()
-
-        # --- plain one-instance-per-launch FP64 path --------------------------
-        del batch
+    mlups_m = None
+    if "mixed" in legs:
+        # --- batched mixed precision and its accuracy against FP64 ----------
+        solvers, batch, st_m, sec_m, kt_m = batched_leg(members, "mixed", args.steps, args.warmup,
+                                                        args.timing_steps)
+        mlups_m = cells * args.steps / sec_m / 1e6
+        batch.close()
         for s in solvers:
             s.close()
+        acc = accuracy(members, args.accuracy_steps, args.accuracy_instances)
+        result["batched_mixed"] = {
+            "mlups": round(mlups_m, 2), "status": st_m,
+            "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kt_m.items()},
+            "rel_l2_vs_fp64": acc,
+            "meets_1e-6": bool(max(acc["rho"], acc["u"], acc["T"]) <= 1e-6),
+        }
+        log(f"batched mixed: {mlups_m:.1f} MLUPS; vs fp64 after {acc['steps']} steps: {acc}")
+
+    if "plain" in legs:
+        # --- plain one-instance-per-launch FP64 path --------------------------
         plain = [lbm.Solver(c, "fp64") for c in members]
         for s in plain:
             s.step(2)
         t0 = time.perf_counter()
-        dev_ms = 0.0
         for s in plain:
             s.step(args.plain_steps)
-            dev_ms += lbm.last_timing()["total_ms"]
         wall = time.perf_counter() - t0
         plain_mlups = cells * args.plain_steps / wall / 1e6
         result["plain_fp64"] = {
             "mlups": round(plain_mlups, 2), "steps_sampled": args.plain_steps,
-            "device_mlups": round(cells * args.plain_steps / (dev_ms * 1e-3) / 1e6, 2),
-            "note": "each instance stepped by its own lbm_step launches; wall clock over the sample",
+            "note": "each instance stepped alone by its own lbm_step launches; host clock over the sample",
         }
-        result["batched_mixed_vs_plain_fp64"] = round(mlups / plain_mlups, 2)
-        log(f"plain fp64: {plain_mlups:.1f} MLUPS (device-time {result['plain_fp64']['device_mlups']})")
+        result["batched_fp64_vs_plain_fp64"] = round(mlups / plain_mlups, 2)
+        if mlups_m is not None:
+            result["batched_mixed_vs_plain_fp64"] = round(mlups_m / plain_mlups, 2)
+        log(f"plain fp64: {plain_mlups:.1f} MLUPS")
         for s in plain:
             s.close()
 
+    if "riemann" in legs:
         # --- 4096^2 Riemann, plain FP64 ------------------------------------
         rc = cases.riemann(args.riemann_n, steps=args.riemann_steps)
         rs = lbm.Solver(rc, "fp64")
         rs.step(2)
-        lbm.set_kernel_timing(True)
+        t0 = time.perf_counter()
         rs.step(args.riemann_steps)
-        rt = lbm.last_timing()
+        r_sec = time.perf_counter() - t0
+        lbm.set_kernel_timing(True)
+        rs.step(min(args.riemann_steps, 5))
+        rk = lbm.last_timing()["kernels"]
         lbm.set_kernel_timing(False)
-        r_mlups = rc.cells * args.riemann_steps / (rt["total_ms"] * 1e-3) / 1e6
-        kb64 = kernel_bytes_per_lup(8)
-        rk = rt["kernels"]
-        rdom = max(("s12", "flux2", "s3", "flux3"), key=lambda k: rk.get(k, {}).get("avg_ms", 0.0))
-        r_ach = kb64[rdom] * rc.cells / (rk[rdom]["avg_ms"] * 1e-3) / 1e9
+        r_mlups = rc.cells * args.riemann_steps / r_sec / 1e6
         result["riemann_fp64"] = {
             "n": args.riemann_n, "steps": args.riemann_steps, "mlups": round(r_mlups, 2),
             "step_roofline_frac": round(r_mlups * 1040.0 / 1e3 / HBM_PEAK_GBS, 4),
-            "dominant_kernel": rdom, "dominant_achieved_gbs": round(r_ach, 1),
+            "roofline": roofline(rk, rc.cells, 8, f"riemann{args.riemann_n}_fp64"),
             "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in rk.items()},
         }
         log(f"riemann {args.riemann_n}^2 fp64: {r_mlups:.1f} MLUPS")
         rs.close()
 
+    if "serve" in legs and args.serve_seconds > 0:
+        # --- served request stream ---------------------------------------
+        result["serve"] = serve_leg(args.serve_rate, args.serve_seconds, 7, args.serve_tick, args.serve_max_wall)
+        log(f"serve: {result['serve']}")
+
+    if "cpu" in legs:
         # --- CPU baseline: the oracle on this host ---------------------------
         try:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             from pyoracle import Oracle
-            sample = members[:]
-            sample.sort(key=lambda c: c.cells)
+            sample = sorted(members, key=lambda c: c.cells)
             c0 = sample[len(sample) // 2]
             o = Oracle(c0)
             done = 0
@@ -246,7 +347,7 @@ def main() -> int:
                           f"instance for {done} steps ({cpu_s:.1f} s); host nproc={os.cpu_count()}",
             }
             o.close()
-        except Exception as e:  # the oracle is optional on a box without it built
+        except Exception as e:  # the oracle library is built by __graft_entry__.build()
             result["cpu_baseline"] = {"value": None, "error": str(e)}
 
     print(json.dumps(result))

@@ -46,6 +46,14 @@ FIXTURES = {
 }
 FIELDS = ("rho", "ux", "uy", "T")
 
+# name -> (builder, args, kwargs, max_steps): decks on which the scheme itself
+# is unstable; the fixture records the step at which the oracle's
+# equilibrium check trips. (Ensemble member 84 with the lattice temperature
+# equal to the flow temperature: a Mach-0.42 Couette deck.)
+FAILURES = {
+    "couette64_uq84_tc1_failure": ("uq_member", [84], {"tc_tref": 1.0}, 5000),
+}
+
 
 def build_case(spec: str) -> "cases.Case":
     d = json.loads(spec)
@@ -53,7 +61,7 @@ def build_case(spec: str) -> "cases.Case":
 
 
 def spec_of(name: str) -> str:
-    fn, args, kwargs = FIXTURES[name][:3]
+    fn, args, kwargs = (FIXTURES.get(name) or FAILURES[name])[:3]
     return json.dumps({"fn": fn, "args": args, "kwargs": kwargs}, sort_keys=True)
 
 
@@ -105,6 +113,23 @@ def make(name: str) -> str:
     return path
 
 
+def make_failure(name: str) -> str:
+    from pyoracle import Oracle
+    spec = spec_of(name)
+    max_steps = FAILURES[name][3]
+    case = build_case(spec)
+    t = time.time()
+    o = Oracle(case)
+    st = o.step(max_steps)
+    fail_step = int(o.consts()["step"]) if st else -1
+    o.close()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, spec=np.array(spec), max_steps=np.int64(max_steps), fail_step=np.int64(fail_step))
+    print(f"{name}: {case.nx}x{case.ny}, oracle fails at step {fail_step}, {time.time() - t:.1f}s "
+          f"-> {os.path.relpath(path, ROOT)}")
+    return path
+
+
 if __name__ == "__main__":
-    for n in (sys.argv[1:] or FIXTURES):
-        make(n)
+    for n in (sys.argv[1:] or list(FIXTURES) + list(FAILURES)):
+        make(n) if n in FIXTURES else make_failure(n)

@@ -117,6 +117,7 @@ def test_case_builders_are_consistent():
         assert 0.05 <= c.meta["u_wall"] <= 0.2 and 0.1 <= c.meta["mach"] <= 0.5
         T = c.meta["T"]
         assert T == pytest.approx(c.meta["u_wall"] ** 2 / (1.4 * c.meta["mach"] ** 2))
+    assert all(c.params["Tc_Tref"] == 2.5 for c in ens)
     reqs = cases.request_stream(rate=50, duration=4.0, seed=1)
     assert 100 < len(reqs) < 320
     assert all(64 <= r.case.nx <= 1024 and 500 <= r.case.nsteps <= 5000 for r in reqs)

@@ -41,3 +41,10 @@ def test_couette_fixture_is_x_invariant_and_sheared():
     prof = ux[0, 1:-1]
     assert np.all(np.diff(prof) > -0.01) and prof[-1] - prof[0] > 0.05
     assert 0 < float(z["floor"]) < 1e-11
+
+
+def test_failure_fixture_is_recorded():
+    z = _load("couette64_uq84_tc1_failure")
+    assert 0 < int(z["fail_step"]) < int(z["max_steps"])
+    case = make_golden.build_case(str(z["spec"]))
+    assert case.params["Tc_Tref"] == 1.0 and case.meta["mach"] > 0.4

@@ -40,3 +40,23 @@ def test_gpu_matches_fixture(name, gpu_lib):
     if make_golden.FIXTURES[name][4]:
         # x-invariant decks stay x-invariant on the GPU
         assert np.all(m["ux"] == m["ux"][:1, :])
+
+
+@pytest.mark.parametrize("name", sorted(make_golden.FAILURES))
+def test_gpu_fails_at_oracle_step(name, gpu_lib):
+    """A deck on which the scheme itself blows up: the GPU FP64 path trips
+    the equilibrium check at the same step as the oracle (thousands of
+    steps of agreement before it)."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    case = make_golden.build_case(str(z["spec"]))
+    fail_step = int(z["fail_step"])
+    assert fail_step > 0
+    print(f"[golden] {name} {case.nx}x{case.ny}: oracle fails at step {fail_step}", flush=True)
+    s = gpu_lib.Solver(case, "fp64")
+    try:
+        st = s.step(int(z["max_steps"]))
+        step = s.info()["step"]
+    finally:
+        s.close()
+    print(f"[golden] {name}: GPU status {st} at step {step}", flush=True)
+    assert st == 1 and step == fail_step

@@ -233,9 +233,15 @@ def uq_sample(rng: np.random.Generator) -> dict:
 
 
 def uq_instance(kind: str, n: int, s: dict, steps: int, gamma: float = 1.4,
-                R: float = 1.0, cfl: float = 0.3) -> Case:
+                R: float = 1.0, cfl: float = 0.3, tc_tref: float = 2.5) -> Case:
     """Map a parameter draw to a deck: T from the wall speed and Mach number,
-    T = u_wall^2/(gamma*R*Ma^2)."""
+    T = u_wall^2/(gamma*R*Ma^2), and the lattice temperature Tc = tc_tref*T.
+
+    tc_tref = 2.5 is the ratio of the reference's own deck
+    (Items/11.04.06-BoundaryLayer-Jacobs/input.mat). With Tc = T (ratio 1)
+    the scheme itself is unstable for the faster walls: 25 of the 256
+    ensemble Couette instances (Mach > 0.31) blow up within 5000 steps, in
+    the CPU oracle and on the GPU at the same step; at 1.5 or 2.5 none do."""
     T = s["u_wall"] ** 2 / (gamma * R * s["mach"] ** 2)
     if kind == "couette":
         c = couette(n=n, u_wall=s["u_wall"], T=T, tau=s["tau"], Pr=s["Pr"],
@@ -245,13 +251,15 @@ def uq_instance(kind: str, n: int, s: dict, steps: int, gamma: float = 1.4,
                        gamma=gamma, R=R, cfl=cfl, steps=steps)
     else:
         raise ValueError(kind)
+    c.params["Tc_Tref"] = float(tc_tref)
     c.meta.update(s)
     c.meta["mach"] = s["mach"]
+    c.meta["tc_tref"] = float(tc_tref)
     return c
 
 
 def uq_ensemble(n_instances: int = 256, seed: int = 2026, sides=(64, 128, 256, 512),
-                steps: int = 5000, kinds=("couette", "shear")) -> list[Case]:
+                steps: int = 5000, kinds=("couette", "shear"), tc_tref: float = 2.5) -> list[Case]:
     """The UQ ensemble: square sides drawn uniformly from `sides`, the kind
     (Couette or periodic shear wave) drawn uniformly, parameters from
     uq_sample. Seeded."""
@@ -261,10 +269,16 @@ def uq_ensemble(n_instances: int = 256, seed: int = 2026, sides=(64, 128, 256, 5
         n = int(rng.choice(sides))
         kind = str(rng.choice(kinds))
         s = uq_sample(rng)
-        c = uq_instance(kind, n, s, steps)
+        c = uq_instance(kind, n, s, steps, tc_tref=tc_tref)
         c.meta.update(dict(index=k, seed=seed))
         out.append(c)
     return out
+
+
+def uq_member(index: int, n_instances: int = 256, seed: int = 2026, steps: int = 5000,
+              tc_tref: float = 2.5) -> Case:
+    """Member `index` of uq_ensemble(n_instances, seed, steps, tc_tref=...)."""
+    return uq_ensemble(n_instances, seed=seed, steps=steps, tc_tref=tc_tref)[index]
 
 
 @dataclass
@@ -276,7 +290,8 @@ class Request:
 
 def request_stream(rate: float = 50.0, duration: float = 60.0, seed: int = 7,
                    side_min: int = 64, side_max: int = 1024, steps_min: int = 500,
-                   steps_max: int = 5000, kinds=("couette", "shear")) -> list[Request]:
+                   steps_max: int = 5000, kinds=("couette", "shear"),
+                   tc_tref: float = 2.5) -> list[Request]:
     """Poisson arrivals at `rate` per second over `duration` seconds; each
     request one solve with side log-uniform in [side_min, side_max], steps
     uniform in [steps_min, steps_max], parameters as in the UQ ensemble."""
@@ -292,7 +307,7 @@ def request_stream(rate: float = 50.0, duration: float = 60.0, seed: int = 7,
         steps = int(rng.integers(steps_min, steps_max + 1))
         kind = str(rng.choice(kinds))
         s = uq_sample(rng)
-        c = uq_instance(kind, n, s, steps)
+        c = uq_instance(kind, n, s, steps, tc_tref=tc_tref)
         out.append(Request(tag, t, c))
         tag += 1
     return out
