@@ -49,12 +49,13 @@ def log(msg: str) -> None:
 
 def kernel_bytes_per_lup(es: int) -> dict:
     """Compulsory HBM traffic of each kernel per lattice update (values of
-    `es` bytes): s12 reads the state and writes f2 (52 values); flux2 reads
-    f2 and writes the 24 net-flux planes (50); s3 reads the state and the
-    net fluxes and writes f3 and G (102); flux3 reads f3 and G and writes
-    the state (78). Halo re-reads, the cell-type map and the wall-closure
-    kernels are excluded."""
-    return {"s12": 52 * es, "flux2": 50 * es, "s3": 102 * es, "flux3": 78 * es}
+    `es` bytes; the four moment planes of the stage-2 net fluxes are FP64):
+    s12 reads the state and writes f2 (52 values); flux2 reads f2 and writes
+    the 24 net-flux planes (50) and their 4 moments; s3 reads the state, the
+    net fluxes and their moments and writes f3 and G (102 + 4); flux3 reads
+    f3 and G and writes the state (78). Halo re-reads, the cell-type map and
+    the wall-closure kernels are excluded."""
+    return {"s12": 52 * es, "flux2": 50 * es + 32, "s3": 102 * es + 32, "flux3": 78 * es}
 
 
 def measured_traffic(kernel: str, workload: str):

@@ -43,6 +43,8 @@ struct InstDev {
   const uint8_t* ctype;
   double* alpha;       // wall closure factor, [ny][pitch]
   int* fail;           // per-instance: first failing step (INT_MAX when none)
+  double* dmom;        // 4 FP64 planes: moments of the stage-2 net fluxes of a
+                       // fluid cell (sum dF, sum dF e_x c, sum dF e_y c, sum dH)
   // physics (reference setupVars, cpuLBM.cpp:360-450)
   double dt, dx, dy;
   double rdx, rdy;     // dt/dx, dt/dy

@@ -259,24 +259,18 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
   double f[kQ], h[kQ];
   load26(src, P, o, f, h);
   // stage-1 moments, and the stage-3 moments: those of f - dF(f2)
-  // (macroProp3, cpuLBM.cpp:973-1032), summed as moments of f minus moments
-  // of dF so that every direction can be finished in one pass
-  double rho = 0, rux = 0, ruy = 0, sh = 0, drho = 0, drux = 0, druy = 0, dsh = 0;
+  // (macroProp3, cpuLBM.cpp:973-1032), formed as moments of f minus the
+  // moments of dF that the stage-2 flux kernel summed for this cell, so that
+  // every direction can be finished in one pass
+  double rho = 0, rux = 0, ruy = 0, sh = 0;
 #pragma unroll
   for (int i = 0; i < kQ; i++) {
-    const double exc = lat_x(i) * in.c, eyc = lat_y(i) * in.c;
     rho += f[i];
-    rux += f[i] * exc;
-    ruy += f[i] * eyc;
+    rux += f[i] * (lat_x(i) * in.c);
+    ruy += f[i] * (lat_y(i) * in.c);
     sh += h[i];
-    if (i) {
-      const double dF = double(dfb[P * (i - 1) + o]), dH = double(dfb[P * (12 + i - 1) + o]);
-      drho += dF;
-      drux += dF * exc;
-      druy += dF * eyc;
-      dsh += dH;
-    }
   }
+  const double drho = in.dmom[o], drux = in.dmom[P + o], druy = in.dmom[2 * P + o], dsh = in.dmom[3 * P + o];
   Macro m1, m3;
   finish_macro(in, rho, rux, ruy, sh, m1);
   finish_macro(in, rho - drho, rux - drux, ruy - druy, sh - dsh, m3);
@@ -541,6 +535,7 @@ struct FluxStage {
   TC h[kRY][kRX];
   TC fxf[kTY][kTX + 1], fxh[kTY][kTX + 1];   // x faces: owner column lx - lo
   TC fyf[kTY + 1][kTX], fyh[kTY + 1][kTX];   // y faces: owner row ly - lo
+  double mom[4][kNT];                        // per-thread sums of the stored dF, dH
 };
 
 // Reference stencil solid handling along one line (cpuLBM.cpp:1242-1322):
@@ -634,7 +629,7 @@ __device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& 
 //   MODE 1: source f3; new state (holding G) -= dF/2, dH/2.
 //   MODE 2: source the current state (EULER, EULER_EQ); dF, dH -> dF planes.
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int MODE>
+template <typename TS, typename TC, int MODE, bool MOM>
 __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
                                           int par) {
   const int t = threadIdx.x;
@@ -676,6 +671,12 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
     s_lds[j] = (ly + kH) * kRX + (lx + kH);
     s_off[j] = kind < 3 ? R.off[ly + kH][lx + kH] : 0;
     s_meta[j] = kind < 3 ? (int)R.meta[ly + kH][lx + kH] | (kind << 8) : (3 << 8);
+  }
+  // MOM: each thread sums the moments of its own cell's stored dF, dH over
+  // the directions in its LDS slots (kept out of registers: occupancy)
+  if (MOM) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) S.mom[k][t] = 0.0;
   }
   for (int I = 1; I < kQ; I++) {
     const int ex = c_lat_x[I], ey = c_lat_y[I];
@@ -755,6 +756,13 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
       if (MODE != 1) {
         dst[P * (I - 1) + oo] = TS(dF);
         dst[P * (12 + I - 1) + oo] = TS(dH);
+        if (MOM) {
+          const double dFs = double(TS(dF));
+          S.mom[0][t] += dFs;
+          S.mom[1][t] += dFs * (ex * in.c);
+          S.mom[2][t] += dFs * (ey * in.c);
+          S.mom[3][t] += double(TS(dH));
+        }
       } else {
         TS* pf = dst + P * I + oo;
         TS* ph = dst + P * (kQ + I) + oo;
@@ -763,13 +771,17 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
       }
     }
   }
+  if (MOM && fluid) {
+    in.dmom[oo] = S.mom[0][t];
+    in.dmom[P + oo] = S.mom[1][t];
+    in.dmom[2 * P + oo] = S.mom[2][t];
+    in.dmom[3 * P + oo] = S.mom[3][t];
+  }
 }
 
-template <typename TS, typename TC, int MODE>
-__global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                              int step) {
-  __shared__ Region R;
-  __shared__ FluxStage<TC> S;
+template <typename TS, typename TC, int MODE, bool MOM>
+__device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                           int step, Region& R, FluxStage<TC>& S) {
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -777,9 +789,41 @@ __global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRe
     __syncthreads();
     setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
     __syncthreads();
-    flux_tile<TS, TC, MODE>(S, R, in, tr, par);
+    flux_tile<TS, TC, MODE, MOM>(S, R, in, tr, par);
   }
 }
+
+// FP32 storage: the compiler's own register allocation (up to 7 waves per
+// SIMD). FP64: capped at 80 registers for 6 waves per SIMD, which the
+// moment-summing stage-2 variant would otherwise miss (83 registers, 5
+// waves; 4096^2: 4.18 -> 3.88 ms per launch).
+template <typename TS, typename TC, int MODE, bool MOM = false>
+__global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                              int step) {
+  __shared__ Region R;
+  __shared__ FluxStage<TC> S;
+  flux_tiles<TS, TC, MODE, MOM>(insts, tiles, ntiles, par, step, R, S);
+}
+
+template <int MODE, bool MOM = false>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6))) void k_flux64(const InstDev* insts,
+                                                                                       const TileRef* tiles,
+                                                                                       int ntiles, int par,
+                                                                                       int step) {
+  __shared__ Region R;
+  __shared__ FluxStage<double> S;
+  flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, step, R, S);
+}
+
+// The flux kernel of a storage/compute type pair.
+template <typename TS, typename TC, int MODE, bool MOM = false>
+struct FluxKernel {
+  static constexpr auto fn = &k_flux<TS, TC, MODE, MOM>;
+};
+template <int MODE, bool MOM>
+struct FluxKernel<double, double, MODE, MOM> {
+  static constexpr auto fn = &k_flux64<MODE, MOM>;
+};
 
 // ---------------------------------------------------------------------------
 // k_alpha: wall closure factor of each wall cell with a non-solid axis
@@ -955,9 +999,9 @@ template <typename TS, typename TC>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
   static int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
   if (!g0) g0 = grid_for((const void*)k_s12<TS, TC>);
-  if (!g1) g1 = grid_for((const void*)k_flux<TS, TC, 0>);
+  if (!g1) g1 = grid_for((const void*)FluxKernel<TS, TC, 0, true>::fn);
   if (!g2) g2 = grid_for((const void*)k_s3<TS, TC>);
-  if (!g3) g3 = grid_for((const void*)k_flux<TS, TC, 1>);
+  if (!g3) g3 = grid_for((const void*)FluxKernel<TS, TC, 1>::fn);
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
@@ -969,7 +1013,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, int step, Ke
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  k_flux<TS, TC, 0><<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, 0, true>::fn<<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   k_s3<TS, TC><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
@@ -980,7 +1024,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, int step, Ke
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(4, s);
-  k_flux<TS, TC, 1><<<blocks_for(n, g3), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, 1>::fn<<<blocks_for(n, g3), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(4, s);
 }
 
@@ -990,7 +1034,7 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, int st
   static int g0 = 0, g1 = 0, g2 = 0;
   constexpr int FM = METHOD == 0 ? 0 : 2;
   if (!g0) g0 = grid_for((const void*)k_s1<TS, TC>);
-  if (!g1) g1 = grid_for((const void*)k_flux<TS, TC, FM>);
+  if (!g1) g1 = grid_for((const void*)FluxKernel<TS, TC, FM>::fn);
   if (!g2) g2 = grid_for((const void*)k_fin<TS, TC, METHOD>);
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
@@ -1005,7 +1049,7 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, int st
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  k_flux<TS, TC, FM><<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, FM>::fn<<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   k_fin<TS, TC, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);

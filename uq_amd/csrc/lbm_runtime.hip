@@ -319,6 +319,7 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.ctype = s.ctype;
   d.alpha = s.alpha;
   d.fail = s.fail;
+  d.dmom = s.macro_dev;  // planes 0-3 of the output buffer, free during a step
   d.dt = s.dt;
   d.dx = s.p.dx;
   d.dy = s.p.dy;
