@@ -183,6 +183,15 @@ int lbm_set_kernel_timing(int32_t enable);
 int lbm_get_timing(lbm_timing* out);
 
 /*
+ * Step sequences (lbm_step, lbm_batch_step, server ticks) replay captured
+ * HIP graphs of one and two steps, recaptured when a group's work lists
+ * change; enable = 0 launches every kernel directly instead (also the case
+ * while kernel timing is on). Results are identical either way. No
+ * counterpart in the reference (whose step is a host loop); on by default.
+ */
+int lbm_set_graphs(int32_t enable);
+
+/*
  * Continuous-batching server. Requests are queued host-side; a request is
  * admitted (device state allocated and initialised) when a slot is free and
  * its device footprint fits within hbm_budget_bytes. Each tick advances

@@ -223,7 +223,8 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, TS* d
 
 template <typename TS, typename TC>
 __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                             int step) {
+                                             const int* stepp) {
+  const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -321,7 +322,9 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
 template <typename TS, typename TC>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k_s3(const InstDev* insts,
                                                                                    const TileRef* tiles,
-                                                                                   int ntiles, int par, int step) {
+                                                                                   int ntiles, int par,
+                                                                                   const int* stepp) {
+  const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -409,7 +412,8 @@ __device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, TS* ds
 
 template <typename TS, typename TC>
 __global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                            int step) {
+                                            const int* stepp) {
+  const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -476,7 +480,8 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
 
 template <typename TS, typename TC, int METHOD>
 __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                             int step) {
+                                             const int* stepp) {
+  const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -799,20 +804,20 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
 // waves; 4096^2: 4.18 -> 3.88 ms per launch).
 template <typename TS, typename TC, int MODE, bool MOM = false>
 __global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                              int step) {
+                                              const int* stepp) {
   __shared__ Region R;
   __shared__ FluxStage<TC> S;
-  flux_tiles<TS, TC, MODE, MOM>(insts, tiles, ntiles, par, step, R, S);
+  flux_tiles<TS, TC, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S);
 }
 
 template <int MODE, bool MOM = false>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6))) void k_flux64(const InstDev* insts,
                                                                                        const TileRef* tiles,
                                                                                        int ntiles, int par,
-                                                                                       int step) {
+                                                                                       const int* stepp) {
   __shared__ Region R;
   __shared__ FluxStage<double> S;
-  flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, step, R, S);
+  flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S);
 }
 
 // The flux kernel of a storage/compute type pair.
@@ -888,7 +893,9 @@ __device__ TC ref_face_flux_f(const InstDev& in, const TS* src, int x, int y, in
 }
 
 template <typename TS, typename TC>
-__global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, int which, int par, int step) {
+__global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, int which, int par,
+                        const int* stepp) {
+  const int step = *stepp;
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwalls) return;
   const WallRef wr = walls[w];
@@ -995,13 +1002,36 @@ static int grid_for(const void* kernel) {
 
 static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 
+// Persistent grid sizes of every step kernel of one storage/compute type,
+// computed once (before any stream capture).
 template <typename TS, typename TC>
-static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
-  static int g0 = 0, g1 = 0, g2 = 0, g3 = 0;
-  if (!g0) g0 = grid_for((const void*)k_s12<TS, TC>);
-  if (!g1) g1 = grid_for((const void*)FluxKernel<TS, TC, 0, true>::fn);
-  if (!g2) g2 = grid_for((const void*)k_s3<TS, TC>);
-  if (!g3) g3 = grid_for((const void*)FluxKernel<TS, TC, 1>::fn);
+struct StepGrids {
+  int s12, flux2, s3, flux3, s1, flux0, fluxe, fin0, fin2, fin3;
+  static const StepGrids& get() {
+    static const StepGrids g = {grid_for((const void*)k_s12<TS, TC>),
+                                grid_for((const void*)FluxKernel<TS, TC, 0, true>::fn),
+                                grid_for((const void*)k_s3<TS, TC>),
+                                grid_for((const void*)FluxKernel<TS, TC, 1>::fn),
+                                grid_for((const void*)k_s1<TS, TC>),
+                                grid_for((const void*)FluxKernel<TS, TC, 0>::fn),
+                                grid_for((const void*)FluxKernel<TS, TC, 2>::fn),
+                                grid_for((const void*)k_fin<TS, TC, 0>),
+                                grid_for((const void*)k_fin<TS, TC, 2>),
+                                grid_for((const void*)k_fin<TS, TC, 3>)};
+    return g;
+  }
+};
+
+void prepare_launch(int fp64) {
+  if (fp64) (void)StepGrids<double, double>::get();
+  else (void)StepGrids<float, float>::get();
+}
+
+template <typename TS, typename TC>
+static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
+  const int* step = L.step_ctr;
+  const StepGrids<TS, TC>& G = StepGrids<TS, TC>::get();
+  const int g0 = G.s12, g1 = G.flux2, g2 = G.s3, g3 = G.flux3;
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
@@ -1030,12 +1060,12 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, int step, Ke
 
 // RKMETHOD 0, 2, 3: one flux stage.
 template <typename TS, typename TC, int METHOD>
-static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
-  static int g0 = 0, g1 = 0, g2 = 0;
+static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
+  const int* step = L.step_ctr;
   constexpr int FM = METHOD == 0 ? 0 : 2;
-  if (!g0) g0 = grid_for((const void*)k_s1<TS, TC>);
-  if (!g1) g1 = grid_for((const void*)FluxKernel<TS, TC, FM>::fn);
-  if (!g2) g2 = grid_for((const void*)k_fin<TS, TC, METHOD>);
+  const StepGrids<TS, TC>& G = StepGrids<TS, TC>::get();
+  const int g0 = G.s1, g1 = FM == 0 ? G.flux0 : G.fluxe;
+  const int g2 = METHOD == 0 ? G.fin0 : (METHOD == 2 ? G.fin2 : G.fin3);
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if constexpr (METHOD == 0) {
@@ -1056,14 +1086,21 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, int st
   if (tm) tm->end(3, s);
 }
 
+// Advances the step counter of a launch sequence: the kernels of step k read
+// k + 1 from it, so one captured sequence replays for any step.
+__global__ void k_tick(int* ctr) {
+  if (threadIdx.x == 0) ctr[0] = ctr[0] + 1;
+}
+
 template <typename TS, typename TC>
-void launch_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm) {
+void launch_step(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   switch (L.rk_method) {
-    case 0: launch_one_stage<TS, TC, 0>(L, s, par, step, tm); break;
-    case 2: launch_one_stage<TS, TC, 2>(L, s, par, step, tm); break;
-    case 3: launch_one_stage<TS, TC, 3>(L, s, par, step, tm); break;
-    default: launch_rk3<TS, TC>(L, s, par, step, tm); break;
+    case 0: launch_one_stage<TS, TC, 0>(L, s, par, tm); break;
+    case 2: launch_one_stage<TS, TC, 2>(L, s, par, tm); break;
+    case 3: launch_one_stage<TS, TC, 3>(L, s, par, tm); break;
+    default: launch_rk3<TS, TC>(L, s, par, tm); break;
   }
+  k_tick<<<1, 64, 0, s>>>(L.step_ctr);
 }
 
 template <typename TS>
@@ -1079,8 +1116,8 @@ void launch_macro(const InstDev* insts, int inst, int par, double* out, long lon
   k_macro<TS><<<blocks, 256, 0, s>>>(insts, inst, par, out);
 }
 
-template void launch_step<double, double>(const StepLaunch&, hipStream_t, int, int, KernelTimer*);
-template void launch_step<float, float>(const StepLaunch&, hipStream_t, int, int, KernelTimer*);
+template void launch_step<double, double>(const StepLaunch&, hipStream_t, int, KernelTimer*);
+template void launch_step<float, float>(const StepLaunch&, hipStream_t, int, KernelTimer*);
 template void launch_init<double>(const InstDev*, int, const int*, const double*, long long, hipStream_t);
 template void launch_init<float>(const InstDev*, int, const int*, const double*, long long, hipStream_t);
 template void launch_macro<double>(const InstDev*, int, int, double*, long long, hipStream_t);

@@ -22,7 +22,14 @@ struct StepLaunch {
   const TileRef* ptiles = nullptr;  // 64 x 4 row tiles of the pointwise kernels
   int nptiles = 0;
   int rk_method = 1;                // RKMETHOD shared by every instance of the group
+  int* step_ctr = nullptr;          // device step counter: step k of a sequence reads k + 1
 };
+
+inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
+  return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
+         a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
+         a.step_ctr == b.step_ctr;
+}
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
 // around each launch; elapsed times are collected after a synchronise.
@@ -76,8 +83,14 @@ int tile_halo();
 int dflux_planes();
 constexpr int kPointW = 64, kPointH = 4;  // pointwise tile: 64 x 4 cells, 256 threads
 
+// One step of every instance of L (buffer parity `par` relative to each
+// instance's parity0), then the step counter advance.
 template <typename TS, typename TC>
-void launch_step(const StepLaunch& L, hipStream_t s, int par, int step, KernelTimer* tm);
+void launch_step(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm);
+
+// Computes the persistent grid sizes of the step kernels (call before a
+// stream capture).
+void prepare_launch(int fp64);
 
 template <typename TS>
 void launch_init(const InstDev* insts, int inst, const int* ids, const double* props, long long cells,

@@ -291,8 +291,10 @@ size_t uqlbm::solver_footprint(int nx, int ny, int precision) {
 }
 
 Solver::~Solver() {
+  if (graphs) graphs->clear();
   for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
-                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux, (void*)ptiles_dev})
+                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
+                  (void*)ptiles_dev, (void*)step_ctr})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -467,6 +469,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   HIP_TRY(hipMalloc((void**)&s->props_dev, s->props.size() * 4 * sizeof(double)));
   HIP_TRY(hipMalloc((void**)&s->inst_dev, sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&s->macro_dev, plane * 5 * sizeof(double)));
+  HIP_TRY(hipMalloc((void**)&s->step_ctr, sizeof(int)));
   s->device_bytes = solver_footprint(nx, ny, precision);
 
   std::vector<uint8_t> ct(plane, 2);
@@ -556,9 +559,45 @@ static void launch_steps(const StepLaunch& L, int precision, hipStream_t stream,
                          KernelTimer* tm) {
   for (int k = 0; k < nsteps; k++) {
     const int par = k & 1;
-    if (precision == LBM_PREC_FP64) launch_step<double, double>(L, stream, par, k + 1, tm);
-    else launch_step<float, float>(L, stream, par, k + 1, tm);
+    if (precision == LBM_PREC_FP64) launch_step<double, double>(L, stream, par, tm);
+    else launch_step<float, float>(L, stream, par, tm);
   }
+}
+
+// HIP graph replay of the step sequence (on unless lbm_set_graphs(0)).
+static bool g_graphs_enabled = true;
+
+static int capture_steps(const StepLaunch& L, int precision, hipStream_t stream, int nsteps,
+                         hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  launch_steps(L, precision, stream, nsteps, nullptr);
+  const hipError_t e = hipStreamEndCapture(stream, &g);
+  if (e != hipSuccess) return fail_with(LBM_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+  const hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) return fail_with(LBM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e2));
+  return LBM_OK;
+}
+
+// n steps of L through the cached graphs (captured on first use or when L
+// changed): the two-step graph n/2 times, then the one-step graph.
+static int replay_steps(GraphCache& gc, const StepLaunch& L, int precision, hipStream_t stream, int n) {
+  if (gc.precision != precision || !same_launch(gc.key, L)) {
+    gc.clear();
+    prepare_launch(precision == LBM_PREC_FP64);
+    int st = capture_steps(L, precision, stream, 2, &gc.two);
+    if (st == LBM_OK) st = capture_steps(L, precision, stream, 1, &gc.one);
+    if (st != LBM_OK) {
+      gc.clear();
+      return st;
+    }
+    gc.key = L;
+    gc.precision = precision;
+  }
+  for (int k = 0; k + 1 < n; k += 2) HIP_TRY(hipGraphLaunch(gc.two, stream));
+  if (n & 1) HIP_TRY(hipGraphLaunch(gc.one, stream));
+  return LBM_OK;
 }
 
 static KernelTimer* active_timer() {
@@ -591,16 +630,26 @@ static void record_timing(double total_ms, KernelTimer* tm) {
 // account each member's steps, stopping a member at its failing step
 // (reference main loop stop on runOne failure, main.cpp:21,56,73).
 int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, const StepLaunch& L, int precision,
-              hipStream_t stream, int n) {
+              hipStream_t stream, int n, GraphCache* graphs) {
+  static const int kFirstStep = 1;
   std::vector<InstDev> table(members.size());
   for (size_t k = 0; k < members.size(); k++) fill_inst(*members[k], table[k]);
   HIP_TRY(hipMemcpyAsync(table_dev, table.data(), table.size() * sizeof(InstDev), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(L.step_ctr, &kFirstStep, sizeof(int), hipMemcpyHostToDevice, stream));
   KernelTimer* tm = active_timer();
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, stream));
-  launch_steps(L, precision, stream, n, tm);
+  if (graphs && g_graphs_enabled && !tm) {
+    const int st = replay_steps(*graphs, L, precision, stream, n);
+    if (st != LBM_OK) {
+      (void)hipStreamSynchronize(stream);
+      return st;
+    }
+  } else {
+    launch_steps(L, precision, stream, n, tm);
+  }
   HIP_TRY(hipEventRecord(e1, stream));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(stream));
@@ -635,11 +684,12 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
   if (s.failed) return LBM_FAILED;
   if (nsteps == 0) return LBM_OK;
-  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles, s.walls_dev, s.nwalls, s.ptiles_dev, s.nptiles, s.p.rk_method};
+  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles,   s.walls_dev, s.nwalls,
+               s.ptiles_dev, s.nptiles, s.p.rk_method, s.step_ctr};
   std::vector<Solver*> one{&s};
   int st = LBM_OK;
   if (want_residual && nsteps > 1) {
-    st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps - 1);
+    st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps - 1, s.graphs.get());
     if (st != LBM_OK) {
       (void)upload_inst(s);
       return st;
@@ -651,7 +701,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
     if (st == LBM_OK) st = macro_into(s, before);
     if (st != LBM_OK) return st;
   }
-  st = run_group(one, s.inst_dev, L, s.precision, s.stream, want_residual ? 1 : nsteps);
+  st = run_group(one, s.inst_dev, L, s.precision, s.stream, want_residual ? 1 : nsteps, s.graphs.get());
   int st2 = upload_inst(s);
   if (st < 0) return st;
   if (st2 != LBM_OK) return st2;
@@ -767,6 +817,11 @@ extern "C" int lbm_set_kernel_timing(int32_t enable) {
   return LBM_OK;
 }
 
+extern "C" int lbm_set_graphs(int32_t enable) {
+  g_graphs_enabled = enable != 0;
+  return LBM_OK;
+}
+
 extern "C" int lbm_get_timing(lbm_timing* out) {
   if (!out) return fail_with(LBM_ERR_ARG, "null argument");
   *out = g_last_timing;
@@ -820,6 +875,7 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   b.nptiles = (int)ptiles.size();
   HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
+  HIP_TRY(hipMalloc((void**)&b.step_ctr, sizeof(int)));
   HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   if (!walls.empty()) {
@@ -833,14 +889,16 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
 }
 
 uqlbm::Batch::~Batch() {
-  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev})
+  if (graphs) graphs->clear();
+  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 int uqlbm::batch_step(Batch& b, int nsteps) {
-  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles, b.walls_dev, b.nwalls, b.ptiles_dev, b.nptiles, b.rk_method};
-  int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps);
+  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles,  b.walls_dev, b.nwalls,
+               b.ptiles_dev, b.nptiles, b.rk_method, b.step_ctr};
+  int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get());
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);
     if (st2 != LBM_OK) return st2;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "../../include/uqlbm.h"
@@ -12,6 +13,25 @@
 #include "lbm_launch.hpp"
 
 namespace uqlbm {
+
+// Instantiated HIP graphs of one launch group's step sequence: two steps
+// (buffer parities 0 and 1, so that it replays back to back) and one step.
+// Valid while the group's launch description is unchanged.
+struct GraphCache {
+  StepLaunch key{};
+  int precision = -1;
+  hipGraphExec_t two = nullptr, one = nullptr;
+  GraphCache() = default;
+  GraphCache(const GraphCache&) = delete;
+  GraphCache& operator=(const GraphCache&) = delete;
+  void clear() {
+    if (two) (void)hipGraphExecDestroy(two);
+    if (one) (void)hipGraphExecDestroy(one);
+    two = one = nullptr;
+    precision = -1;
+  }
+  ~GraphCache() { clear(); }
+};
 
 struct PropRow {
   double rho, p, T, ux, uy;
@@ -49,6 +69,8 @@ struct Solver {
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
   double* macro_dev = nullptr;
+  int* step_ctr = nullptr;          // device step counter of this solver's launches
+  std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   size_t device_bytes = 0;
 
@@ -71,6 +93,8 @@ struct Batch {
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
+  int* step_ctr = nullptr;
+  std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
@@ -83,6 +107,8 @@ struct Batch {
     tiles_dev = o.tiles_dev;
     walls_dev = o.walls_dev;
     ptiles_dev = o.ptiles_dev;
+    step_ctr = o.step_ctr;
+    graphs = std::move(o.graphs);
     ntiles = o.ntiles;
     nwalls = o.nwalls;
     nptiles = o.nptiles;
@@ -91,6 +117,7 @@ struct Batch {
     o.tiles_dev = nullptr;
     o.walls_dev = nullptr;
     o.ptiles_dev = nullptr;
+    o.step_ctr = nullptr;
     return *this;
   }
   ~Batch();
@@ -102,6 +129,6 @@ int upload_inst(Solver& s);
 int batch_finish(Batch& b, lbm_batch** out);
 int batch_step(Batch& b, int nsteps);
 int run_group(const std::vector<Solver*>& members, InstDev* table_dev, const StepLaunch& L, int precision,
-              hipStream_t stream, int n);
+              hipStream_t stream, int n, GraphCache* graphs);
 
 }  // namespace uqlbm

@@ -65,6 +65,8 @@ struct Server {
   size_t tiles_cap = 0, walls_cap = 0, ptiles_cap = 0;
   int ntiles = 0, nwalls = 0, nptiles = 0;
   bool lists_dirty = true;
+  int* step_ctr = nullptr;
+  GraphCache graphs;  // step graphs of the current active set (recaptured when it changes)
   ~Server();
 };
 
@@ -76,8 +78,9 @@ static double now_s() {
 static size_t footprint_of(int nx, int ny, int precision) { return solver_footprint(nx, ny, precision); }
 
 Server::~Server() {
+  graphs.clear();
   for (auto& a : active) delete a.solver;
-  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev})
+  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -230,6 +233,7 @@ extern "C" int lbm_server_create(int32_t precision, int32_t rk_method, size_t hb
   sv->max_slots = max_slots;
   SV_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
   SV_TRY(hipMalloc((void**)&sv->table_dev, (size_t)max_slots * sizeof(InstDev)));
+  SV_TRY(hipMalloc((void**)&sv->step_ctr, sizeof(int)));
   *out = reinterpret_cast<lbm_server*>(sv.release());
   return LBM_OK;
 }
@@ -282,11 +286,11 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     std::vector<Solver*> members;
     members.reserve(sv.active.size());
     for (auto& a : sv.active) members.push_back(a.solver);
-    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles, sv.walls_dev, sv.nwalls, sv.ptiles_dev, sv.nptiles,
-                 sv.rk_method};
+    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles,   sv.walls_dev, sv.nwalls,
+                 sv.ptiles_dev, sv.nptiles, sv.rk_method, sv.step_ctr};
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
-    st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round);
+    st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round, &sv.graphs);
     if (st < 0) return st;
     if (st == LBM_FAILED) status = LBM_FAILED;
     sv.lups += cells * round;

@@ -58,6 +58,7 @@ def lib():
     L.lbm_batch_destroy.argtypes = [vp]
     L.lbm_batch_destroy.restype = None
     L.lbm_set_kernel_timing.argtypes = [i32]
+    L.lbm_set_graphs.argtypes = [i32]
     L.lbm_get_timing.argtypes = [P(LbmTiming)]
     L.lbm_clock_s.restype = d
     if hasattr(L, "lbm_server_create"):
@@ -86,6 +87,12 @@ def _check(status: int, where: str, allow_failed: bool = False) -> int:
 
 def set_kernel_timing(enable: bool) -> None:
     lib().lbm_set_kernel_timing(1 if enable else 0)
+
+
+def set_graphs(enable: bool) -> None:
+    """Replay captured HIP graphs of the step sequence (default) or launch
+    every kernel directly."""
+    lib().lbm_set_graphs(1 if enable else 0)
 
 
 def last_timing() -> dict:
