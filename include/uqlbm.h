@@ -149,6 +149,14 @@ int lbm_get_dist(lbm_solver* s, double* f, double* h);
  * used for checkpoint/resume and for injecting states in tests. */
 int lbm_set_dist(lbm_solver* s, const double* f, const double* h);
 
+/* Conserved totals over the fluid cells (cellType 0), FP64 sums:
+ * out[0] = sum f (mass), out[1] = sum f e_x, out[2] = sum f e_y (momentum),
+ * out[3] = sum h (total energy), e in physical units (lattice x c). The
+ * reference checks conservation only through its per-cell equilibrium mass
+ * check (cpuLBM.cpp:825-843); these totals let a caller verify that mass,
+ * momentum and energy are conserved on closed (periodic) domains. */
+int lbm_get_totals(lbm_solver* s, double* out);
+
 int lbm_get_info(const lbm_solver* s, lbm_info* out);
 
 void lbm_destroy(lbm_solver* s);

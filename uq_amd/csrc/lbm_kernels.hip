@@ -982,6 +982,80 @@ __global__ void k_macro(const InstDev* insts, int inst, int par, double* outp) {
 }
 
 // ---------------------------------------------------------------------------
+// Conserved totals of one instance over its fluid cells: sum f (mass),
+// sum f e_x c, sum f e_y c (momentum), sum h (total energy), FP64. A
+// deterministic two-pass reduction: every block reduces its cells with
+// 64-lane wavefront shuffles and writes one partial per quantity; one block
+// then sums the partials in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 512;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sums v over the block (256 threads = 4 wavefronts); the result is valid in
+// thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds4) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds4[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+  __syncthreads();
+  return r;
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_totals(const InstDev* insts, int inst, int par, double* partials) {
+  __shared__ double lds4[4];
+  const InstDev& in = insts[inst];
+  const long long n = (long long)in.ny * in.pitch;
+  const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+  double acc[4] = {0, 0, 0, 0};
+  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
+    if ((int)(o % in.pitch) >= in.nx || in.ctype[o] != 0) continue;
+#pragma unroll
+    for (int i = 0; i < kQ; i++) {
+      const double f = double(src[in.plane * i + o]);
+      acc[0] += f;
+      acc[1] += f * (lat_x(i) * in.c);
+      acc[2] += f * (lat_y(i) * in.c);
+      acc[3] += double(src[in.plane * (kQ + i) + o]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double b = block_sum(acc[k], lds4);
+    if (threadIdx.x == 0) partials[k * kRedBlocks + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_totals_final(const double* partials, int nparts, double* out) {
+  __shared__ double lds4[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double v = 0;
+    for (int j = threadIdx.x; j < nparts; j += blockDim.x) v += partials[k * kRedBlocks + j];
+    const double b = block_sum(v, lds4);
+    if (threadIdx.x == 0) out[k] = b;
+  }
+}
+
+template <typename TS>
+void launch_totals(const InstDev* insts, int inst, int par, double* scratch, long long cells, hipStream_t s) {
+  long long b = (cells + 255) / 256;
+  const int blocks = (int)(b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks);
+  k_totals<TS><<<blocks, 256, 0, s>>>(insts, inst, par, scratch);
+  k_totals_final<<<1, 256, 0, s>>>(scratch, blocks, scratch + 4 * kRedBlocks);
+}
+
+int totals_scratch_doubles() { return 4 * kRedBlocks + 4; }
+
+// ---------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------
 int tile_w() { return kTX; }
@@ -1122,5 +1196,7 @@ template void launch_init<double>(const InstDev*, int, const int*, const double*
 template void launch_init<float>(const InstDev*, int, const int*, const double*, long long, hipStream_t);
 template void launch_macro<double>(const InstDev*, int, int, double*, long long, hipStream_t);
 template void launch_macro<float>(const InstDev*, int, int, double*, long long, hipStream_t);
+template void launch_totals<double>(const InstDev*, int, int, double*, long long, hipStream_t);
+template void launch_totals<float>(const InstDev*, int, int, double*, long long, hipStream_t);
 
 }  // namespace uqlbm

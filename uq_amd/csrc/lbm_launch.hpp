@@ -99,4 +99,11 @@ void launch_init(const InstDev* insts, int inst, const int* ids, const double* p
 template <typename TS>
 void launch_macro(const InstDev* insts, int inst, int par, double* out, long long cells, hipStream_t s);
 
+// Conserved totals (mass, x and y momentum, total energy) of the fluid cells
+// of one instance; `scratch` holds totals_scratch_doubles() doubles and the
+// four results land in its last four.
+template <typename TS>
+void launch_totals(const InstDev* insts, int inst, int par, double* scratch, long long cells, hipStream_t s);
+int totals_scratch_doubles();
+
 }  // namespace uqlbm

@@ -294,7 +294,7 @@ Solver::~Solver() {
   if (graphs) graphs->clear();
   for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
                   (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
-                  (void*)ptiles_dev, (void*)step_ctr})
+                  (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -737,6 +737,21 @@ extern "C" int lbm_get_macro(lbm_solver* h, double* rho, double* ux, double* uy,
     for (int x = 0; x < s.nx; x++)
       for (int y = 0; y < s.ny; y++) outs[k][(size_t)x * s.ny + y] = m[k * n + (size_t)y * s.pitch + x];
   }
+  return LBM_OK;
+}
+
+extern "C" int lbm_get_totals(lbm_solver* h, double* out) {
+  if (!h || !out) return fail_with(LBM_ERR_ARG, "null argument");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_get_totals before lbm_init");
+  if (!s.totals_dev) HIP_TRY(hipMalloc((void**)&s.totals_dev, totals_scratch_doubles() * sizeof(double)));
+  const long long cells = (long long)s.pitch * s.ny;
+  if (s.precision == LBM_PREC_FP64) launch_totals<double>(s.inst_dev, 0, 0, s.totals_dev, cells, s.stream);
+  else launch_totals<float>(s.inst_dev, 0, 0, s.totals_dev, cells, s.stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, s.totals_dev + totals_scratch_doubles() - 4, 4 * sizeof(double),
+                         hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
   return LBM_OK;
 }
 

@@ -70,6 +70,7 @@ struct Solver {
   TileRef* ptiles_dev = nullptr;
   double* macro_dev = nullptr;
   int* step_ctr = nullptr;          // device step counter of this solver's launches
+  double* totals_dev = nullptr;     // conserved-totals reduction scratch (allocated on first use)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   size_t device_bytes = 0;

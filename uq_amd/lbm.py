@@ -59,6 +59,7 @@ def lib():
     L.lbm_batch_destroy.restype = None
     L.lbm_set_kernel_timing.argtypes = [i32]
     L.lbm_set_graphs.argtypes = [i32]
+    L.lbm_get_totals.argtypes = [vp, P(d)]
     L.lbm_get_timing.argtypes = [P(LbmTiming)]
     L.lbm_clock_s.restype = d
     if hasattr(L, "lbm_server_create"):
@@ -144,6 +145,12 @@ class Solver:
         _check(lib().lbm_get_macro(self._h, *(dptr(arr[k]) for k in ("rho", "ux", "uy", "p", "T"))),
                "lbm_get_macro")
         return {k: v.reshape(self.nx, self.ny) for k, v in arr.items()}
+
+    def totals(self) -> np.ndarray:
+        """[mass, x momentum, y momentum, total energy] over the fluid cells."""
+        out = np.zeros(4)
+        _check(lib().lbm_get_totals(self._h, dptr(out)), "lbm_get_totals")
+        return out
 
     def get_dist(self):
         n = self.nx * self.ny * LBM_Q
