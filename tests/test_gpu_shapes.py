@@ -1,0 +1,92 @@
+"""Flux tile shapes and the autotuner. Every face flux and net flux is the
+same arithmetic whatever the tiling, so all shapes give bitwise identical
+states (and match the oracle); the autotuner leaves the solution untouched
+and the tuned batch continues exactly as an untuned one."""
+import numpy as np
+import pytest
+
+from uq_amd import cases
+from uq_amd.lbm import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+DECKS = [
+    cases.couette(37, nx=45, u_wall=0.15, steps=12),
+    cases.flat_plate(80, 40, plate_start=20, steps=12),
+    cases.riemann(64, steps=8),
+    cases.shear_wave(32, rk_method=0, steps=12),
+]
+
+
+def _mirrored():
+    c = cases.flat_plate(70, 36, plate_start=10, steps=10)
+    c.params["mirror_north"] = 1
+    c.params["mirror_east"] = 1
+    return c
+
+
+def _state(lbm, case, shape, precision="fp64", steps=None):
+    lbm.set_flux_shape(precision, shape)
+    try:
+        s = lbm.Solver(case, precision)
+        st = s.step(case.nsteps if steps is None else steps)
+        f, h = s.get_dist()
+        s.close()
+        return st, f, h
+    finally:
+        lbm.set_flux_shape(precision, 0)
+
+
+@pytest.mark.parametrize("case", DECKS + [_mirrored()], ids=lambda c: f"{c.name}_rk{c.params['rk_method']}")
+def test_shapes_bitwise_equal(gpu_lib, case):
+    print(f"[shapes] {case.name} dims {[gpu_lib.flux_shape_dims(k) for k in range(gpu_lib.FLUX_SHAPES)]}",
+          flush=True)
+    ref = _state(gpu_lib, case, 0)
+    assert ref[0] == 0
+    for shape in range(1, gpu_lib.FLUX_SHAPES):
+        st, f, h = _state(gpu_lib, case, shape)
+        assert st == 0
+        assert np.array_equal(f, ref[1]) and np.array_equal(h, ref[2]), shape
+
+
+def test_shapes_match_oracle(gpu_lib):
+    from pyoracle import Oracle
+    c = cases.couette(33, nx=70, u_wall=0.12, steps=10)
+    o = Oracle(c)
+    o.step(10)
+    fo, ho = o.get_dist()
+    o.close()
+    for shape in range(gpu_lib.FLUX_SHAPES):
+        st, f, h = _state(gpu_lib, c, shape)
+        print(f"[shapes] shape {shape}: f {rel_l2(f, fo):.2e} h {rel_l2(h, ho):.2e}", flush=True)
+        assert st == 0 and rel_l2(f, fo) <= 1e-12 and rel_l2(h, ho) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_keeps_state_and_continues_identically(gpu_lib, precision):
+    members = cases.uq_ensemble(5, seed=21, sides=(64, 128), steps=30)
+    print(f"[shapes] autotune {precision}", flush=True)
+    a = [gpu_lib.Solver(c, precision) for c in members]
+    b = [gpu_lib.Solver(c, precision) for c in members]
+    ba, bb = gpu_lib.Batch(a), gpu_lib.Batch(b)
+    try:
+        assert ba.step(4) == 0 and bb.step(4) == 0
+        before = [s.get_dist() for s in a]
+        shape, ms = ba.autotune(steps=2)
+        print(f"[shapes] chosen {shape} ms/step {ms}", flush=True)
+        assert 0 <= shape < gpu_lib.FLUX_SHAPES and all(m > 0 for m in ms)
+        for s, (f0, h0) in zip(a, before):
+            f1, h1 = s.get_dist()
+            assert np.array_equal(f0, f1) and np.array_equal(h0, h1)
+            assert s.info()["step"] == 4
+        assert ba.step(6) == 0 and bb.step(6) == 0
+        for sa, sb in zip(a, b):
+            fa, ha = sa.get_dist()
+            fb, hb = sb.get_dist()
+            assert np.array_equal(fa, fb) and np.array_equal(ha, hb)
+    finally:
+        ba.close()
+        bb.close()
+        for s in a + b:
+            s.close()
+        gpu_lib.set_flux_shape(precision, 0)

@@ -31,7 +31,6 @@ enum : int {
 // memory as an array indexed by instance id.
 struct InstDev {
   int nx, ny, pitch, flags;
-  int tiles_x, tiles_y;
   int flux_method;     // 0 NND, 1 WENO5
   int parity0;         // which state buffer holds the current state at launch step 0
   int step_base;       // steps taken before the current launch sequence

@@ -31,11 +31,8 @@
 
 namespace uqlbm {
 
-constexpr int kTX = 15, kTY = 16, kH = 3;
-constexpr int kNT = 256;
-constexpr int kRX = kTX + 2 * kH, kRY = kTY + 2 * kH;
-constexpr int kOwn = kTX * kTY;                // 240 own cells per tile
-constexpr int kNXB = 2 * kH * kTY, kNYB = 2 * kH * kTX;
+constexpr int kH = 3;                          // stencil halo (WENO5 reach)
+constexpr int kNT = 256;                       // threads of the pointwise blocks
 constexpr int kNDF = 24;                       // dF planes: f and h of directions 1..12
 
 // region meta bits
@@ -502,19 +499,47 @@ __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef
 
 // ---------------------------------------------------------------------------
 // Stencil tiles: region (tile + 3-cell halo cross), staging and faces.
+//
+// A tile shape is TX x TY own cells worked by NT threads. Per direction the
+// threads stage the own cells and the halo bands of the direction's nonzero
+// axes into LDS, compute the (TX+1) x TY x faces and TX x (TY+1) y faces
+// (each face once, by the upwind owner's rule), then each own fluid cell's
+// net flux. The shapes below keep the face and cell counts at or below one
+// (or two) per thread so no wave waits on a straggler at the barriers; the
+// autotuner picks one per launch group.
 // ---------------------------------------------------------------------------
+template <int TX_, int TY_, int NT_>
+struct Shape {
+  static constexpr int TX = TX_, TY = TY_, NT = NT_;
+  static constexpr int RX = TX + 2 * kH, RY = TY + 2 * kH;
+  static constexpr int Own = TX * TY;
+  static constexpr int NXB = 2 * kH * TY, NYB = 2 * kH * TX;
+  static constexpr int Slots = (Own + NXB + NYB + NT - 1) / NT;
+  static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
+  static constexpr int XF = (NXF + NT - 1) / NT, YF = (NYF + NT - 1) / NT;
+  static constexpr int Cells = (Own + NT - 1) / NT;
+};
+using Shape0 = Shape<15, 16, 256>;  // 240 own cells, 1.78 staged values per cell
+using Shape1 = Shape<31, 15, 512>;  // 465 own cells, 1.59
+using Shape2 = Shape<15, 32, 512>;  // 480 own cells, 1.59
+static_assert(Shape0::XF == 1 && Shape0::YF == 1 && Shape1::XF == 1 && Shape1::YF == 1 &&
+                  Shape2::XF == 1 && Shape2::YF == 1,
+              "one x face and one y face per thread");
+
+template <typename Sh>
 struct Region {
-  int off[kRY][kRX];
-  unsigned char meta[kRY][kRX];
+  int off[Sh::RY][Sh::RX];
+  unsigned char meta[Sh::RY][Sh::RX];
 };
 
-__device__ void setup_region(Region& R, const InstDev& in, int x0, int y0) {
+template <typename Sh>
+__device__ void setup_region(Region<Sh>& R, const InstDev& in, int x0, int y0) {
   const int px_ = in.flags & kPeriodicX, py_ = in.flags & kPeriodicY;
   const int mw = in.flags & kMirrorW, me = in.flags & kMirrorE;
   const int ms = in.flags & kMirrorS, mn = in.flags & kMirrorN;
-  for (int r = threadIdx.x; r < kRX * kRY; r += kNT) {
-    const int ly = r / kRX - kH, lx = r % kRX - kH;
-    const bool inx = lx >= 0 && lx < kTX, iny = ly >= 0 && ly < kTY;
+  for (int r = threadIdx.x; r < Sh::RX * Sh::RY; r += Sh::NT) {
+    const int ly = r / Sh::RX - kH, lx = r % Sh::RX - kH;
+    const bool inx = lx >= 0 && lx < Sh::TX, iny = ly >= 0 && ly < Sh::TY;
     if (!inx && !iny) {  // corner: never read
       R.off[ly + kH][lx + kH] = 0;
       R.meta[ly + kH][lx + kH] = 0;
@@ -534,13 +559,13 @@ __device__ void setup_region(Region& R, const InstDev& in, int x0, int y0) {
   }
 }
 
-template <typename TC>
+template <typename Sh, typename TC, bool MOM = false>
 struct FluxStage {
-  TC f[kRY][kRX];
-  TC h[kRY][kRX];
-  TC fxf[kTY][kTX + 1], fxh[kTY][kTX + 1];   // x faces: owner column lx - lo
-  TC fyf[kTY + 1][kTX], fyh[kTY + 1][kTX];   // y faces: owner row ly - lo
-  double mom[4][kNT];                        // per-thread sums of the stored dF, dH
+  TC f[Sh::RY][Sh::RX];
+  TC h[Sh::RY][Sh::RX];
+  TC fxf[Sh::TY][Sh::TX + 1], fxh[Sh::TY][Sh::TX + 1];   // x faces: face j is left of cell j
+  TC fyf[Sh::TY + 1][Sh::TX], fyh[Sh::TY + 1][Sh::TX];   // y faces: face j is below cell j
+  double mom[MOM ? 4 : 1][MOM ? Sh::Own : 1];            // per-cell sums of the stored dF, dH (MOM)
 };
 
 // Reference stencil solid handling along one line (cpuLBM.cpp:1242-1322):
@@ -570,9 +595,9 @@ __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) 
 // Face flux (f and h) out of the owner at logical (lx, ly) along axis AX with
 // stencil step sg and speed v = |e_axis|: the wall closure at solid owners,
 // else WENO5/NND on the staged line.
-template <int AX, typename TC>
-__device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& R, const InstDev& in, bool solids,
-                                          int lx, int ly, int sg, TC v, TC& vf, TC& vh) {
+template <int AX, typename Sh, typename TC, bool MOM>
+__device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
+                                          bool solids, int lx, int ly, int sg, TC v, TC& vf, TC& vh) {
   const unsigned mo = R.meta[ly + kH][lx + kH];
   if ((mo & kMetaType) == 2) {
     const TC a = TC(in.alpha[R.off[ly + kH][lx + kH]]);
@@ -593,13 +618,13 @@ __device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& 
       any |= sld[k];
     }
     if (any) {
-      TC Sf[5], Sh[5];
+      TC Ef[5], Eh[5];
 #pragma unroll
-      for (int k = 0; k < 5; k++) { Sf[k] = Vf[k]; Sh[k] = Vh[k]; }
-      extrapolate<5>(sld, Vf, Sf);
-      extrapolate<5>(sld, Vh, Sh);
+      for (int k = 0; k < 5; k++) { Ef[k] = Vf[k]; Eh[k] = Vh[k]; }
+      extrapolate<5>(sld, Vf, Ef);
+      extrapolate<5>(sld, Vh, Eh);
 #pragma unroll
-      for (int k = 0; k < 5; k++) { Vf[k] = Sf[k]; Vh[k] = Sh[k]; }
+      for (int k = 0; k < 5; k++) { Vf[k] = Ef[k]; Vh[k] = Eh[k]; }
     }
     vf = weno5(v, Vf[0], Vf[1], Vf[2], Vf[3], Vf[4]);
     vh = weno5(v, Vh[0], Vh[1], Vh[2], Vh[3], Vh[4]);
@@ -615,13 +640,13 @@ __device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& 
       any |= sld[k];
     }
     if (any) {
-      TC Sf[3], Sh[3];
+      TC Ef[3], Eh[3];
 #pragma unroll
-      for (int k = 0; k < 3; k++) { Sf[k] = Vf[k]; Sh[k] = Vh[k]; }
-      extrapolate<3>(sld, Vf, Sf);
-      extrapolate<3>(sld, Vh, Sh);
+      for (int k = 0; k < 3; k++) { Ef[k] = Vf[k]; Eh[k] = Vh[k]; }
+      extrapolate<3>(sld, Vf, Ef);
+      extrapolate<3>(sld, Vh, Eh);
 #pragma unroll
-      for (int k = 0; k < 3; k++) { Vf[k] = Sf[k]; Vh[k] = Sh[k]; }
+      for (int k = 0; k < 3; k++) { Vf[k] = Ef[k]; Vh[k] = Eh[k]; }
     }
     vf = nnd(v, Vf[0], Vf[1], Vf[2]);
     vh = nnd(v, Vh[0], Vh[1], Vh[2]);
@@ -633,62 +658,66 @@ __device__ __forceinline__ void face_flux(const FluxStage<TC>& S, const Region& 
 //   MODE 0: source f2 (f1 for RK1); dF, dH -> the dF planes.
 //   MODE 1: source f3; new state (holding G) -= dF/2, dH/2.
 //   MODE 2: source the current state (EULER, EULER_EQ); dF, dH -> dF planes.
+// MOM (with MODE 0 of RK3): also the moments of each fluid cell's stored
+// dF, dH over the directions -> in.dmom.
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int MODE, bool MOM>
-__device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, const InstDev& in, const TileRef& tr,
-                                          int par) {
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+__device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
+                                          const TileRef& tr, int par) {
   const int t = threadIdx.x;
-  const int tx = t % 16, ty = t / 16;
-  const int x = tr.tx * kTX + tx, y = tr.ty * kTY + ty;
-  const bool own = tx < kTX && x < in.nx && y < in.ny;
-  const bool fluid = own && (R.meta[ty + kH][tx + kH] & kMetaType) == 0;
-  const long long oo = own ? R.off[ty + kH][tx + kH] : 0;
   const bool solids = tr.flags & kTileSolid;
   const long long P = in.plane;
   void* src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
   TS* dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
   const TC rdx = TC(in.rdx), rdy = TC(in.rdy);
-  // Staging slots of this thread, fixed for the tile: slot r < kOwn is an
-  // own cell, then the x halo bands (kNXB), then the y halo bands (kNYB).
-  // Each slot keeps its LDS position, grid offset and meta byte; a direction
-  // stages the own slots and the bands along its nonzero axes.
-  constexpr int kSlots = (kOwn + kNXB + kNYB + kNT - 1) / kNT;
-  int s_lds[kSlots], s_off[kSlots], s_meta[kSlots];
+  // own cells of this thread: cell ci = t + q NT at (ci % TX, ci / TX)
+  bool fluid[Sh::Cells];
+  int oo[Sh::Cells];
 #pragma unroll
-  for (int j = 0; j < kSlots; j++) {
-    const int r = t + j * kNT;
+  for (int q = 0; q < Sh::Cells; q++) {
+    const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
+    const bool own = ci < Sh::Own && tr.tx * Sh::TX + tx < in.nx && tr.ty * Sh::TY + ty < in.ny;
+    fluid[q] = own && (R.meta[ty + kH][tx + kH] & kMetaType) == 0;
+    oo[q] = own ? R.off[ty + kH][tx + kH] : 0;
+    if (MOM && ci < Sh::Own) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) S.mom[k][ci] = 0.0;
+    }
+  }
+  // Staging slots of this thread, fixed for the tile: slot r < Own is an own
+  // cell, then the x halo bands (NXB), then the y halo bands (NYB). Each slot
+  // keeps its LDS position, grid offset and meta byte; a direction stages the
+  // own slots and the bands along its nonzero axes.
+  int s_lds[Sh::Slots], s_off[Sh::Slots], s_meta[Sh::Slots];
+#pragma unroll
+  for (int j = 0; j < Sh::Slots; j++) {
+    const int r = t + j * Sh::NT;
     int lx = 0, ly = 0, kind = 3;  // 0 own, 1 x band, 2 y band, 3 none
-    if (r < kOwn) {
-      lx = r % kTX;
-      ly = r / kTX;
+    if (r < Sh::Own) {
+      lx = r % Sh::TX;
+      ly = r / Sh::TX;
       kind = 0;
-    } else if (r < kOwn + kNXB) {
-      const int k = r - kOwn, side = k / (kH * kTY), kk = k % (kH * kTY);
+    } else if (r < Sh::Own + Sh::NXB) {
+      const int k = r - Sh::Own, side = k / (kH * Sh::TY), kk = k % (kH * Sh::TY);
       ly = kk / kH;
-      lx = side ? kTX + kk % kH : -kH + kk % kH;
+      lx = side ? Sh::TX + kk % kH : -kH + kk % kH;
       kind = 1;
-    } else if (r < kOwn + kNXB + kNYB) {
-      const int k = r - kOwn - kNXB, side = k / (kH * kTX), kk = k % (kH * kTX);
-      lx = kk % kTX;
-      ly = side ? kTY + kk / kTX : -kH + kk / kTX;
+    } else if (r < Sh::Own + Sh::NXB + Sh::NYB) {
+      const int k = r - Sh::Own - Sh::NXB, side = k / (kH * Sh::TX), kk = k % (kH * Sh::TX);
+      lx = kk % Sh::TX;
+      ly = side ? Sh::TY + kk / Sh::TX : -kH + kk / Sh::TX;
       kind = 2;
     }
-    s_lds[j] = (ly + kH) * kRX + (lx + kH);
+    s_lds[j] = (ly + kH) * Sh::RX + (lx + kH);
     s_off[j] = kind < 3 ? R.off[ly + kH][lx + kH] : 0;
     s_meta[j] = kind < 3 ? (int)R.meta[ly + kH][lx + kH] | (kind << 8) : (3 << 8);
-  }
-  // MOM: each thread sums the moments of its own cell's stored dF, dH over
-  // the directions in its LDS slots (kept out of registers: occupancy)
-  if (MOM) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) S.mom[k][t] = 0.0;
   }
   for (int I = 1; I < kQ; I++) {
     const int ex = c_lat_x[I], ey = c_lat_y[I];
     const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
     const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
 #pragma unroll
-    for (int j = 0; j < kSlots; j++) {
+    for (int j = 0; j < Sh::Slots; j++) {
       const int kind = s_meta[j] >> 8;
       if (kind == 3 || (kind == 1 && !ex) || (kind == 2 && !ey)) continue;
       const unsigned m = (unsigned)s_meta[j];
@@ -698,25 +727,29 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
       (&S.h[0][0])[s_lds[j]] = TC(plane<TS>(src, P, kQ + d)[o]);
     }
     __syncthreads();
-    // faces: thread t owns x face (row ty, owner column tx - [sx > 0]) and
-    // y face (column t % 15, owner row t / 15 - [sy > 0])
-    if (ex) {
+    // faces: x face j of row r (owner column j - [sx > 0]); y face j of
+    // column c (owner row j - [sy > 0])
+    if (ex && t < Sh::NXF) {
+      const int row = t / (Sh::TX + 1), j = t % (Sh::TX + 1);
       const TC v = TC(double(ex > 0 ? ex : -ex) * in.c);
       TC vf, vh;
-      face_flux<0, TC>(S, R, in, solids, tx - (sx > 0 ? 1 : 0), ty, sx, v, vf, vh);
-      S.fxf[ty][tx] = vf;
-      S.fxh[ty][tx] = vh;
+      face_flux<0>(S, R, in, solids, j - (sx > 0 ? 1 : 0), row, sx, v, vf, vh);
+      S.fxf[row][j] = vf;
+      S.fxh[row][j] = vh;
     }
-    if (ey && t < kTX * (kTY + 1)) {
-      const int col = t % kTX, row = t / kTX;
+    if (ey && t < Sh::NYF) {
+      const int col = t % Sh::TX, j = t / Sh::TX;
       const TC v = TC(double(ey > 0 ? ey : -ey) * in.c);
       TC vf, vh;
-      face_flux<1, TC>(S, R, in, solids, col, row - (sy > 0 ? 1 : 0), sy, v, vf, vh);
-      S.fyf[row][col] = vf;
-      S.fyh[row][col] = vh;
+      face_flux<1>(S, R, in, solids, col, j - (sy > 0 ? 1 : 0), sy, v, vf, vh);
+      S.fyf[j][col] = vf;
+      S.fyh[j][col] = vh;
     }
     __syncthreads();
-    if (fluid) {
+#pragma unroll
+    for (int q = 0; q < Sh::Cells; q++) {
+      if (!fluid[q]) continue;
+      const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
       // reference cFLUX (cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) +
       // (dt/dy)(Fy_out - Fy_in), grouped out - in; an upwind owner across a
       // clamped edge is the cell itself
@@ -758,76 +791,81 @@ __device__ __forceinline__ void flux_tile(FluxStage<TC>& S, const Region& R, con
         dF = rdy * oyf - rdy * iyf;
         dH = rdy * oyh - rdy * iyh;
       }
+      const long long o = oo[q];
       if (MODE != 1) {
-        dst[P * (I - 1) + oo] = TS(dF);
-        dst[P * (12 + I - 1) + oo] = TS(dH);
+        dst[P * (I - 1) + o] = TS(dF);
+        dst[P * (12 + I - 1) + o] = TS(dH);
         if (MOM) {
           const double dFs = double(TS(dF));
-          S.mom[0][t] += dFs;
-          S.mom[1][t] += dFs * (ex * in.c);
-          S.mom[2][t] += dFs * (ey * in.c);
-          S.mom[3][t] += double(TS(dH));
+          S.mom[0][ci] += dFs;
+          S.mom[1][ci] += dFs * (ex * in.c);
+          S.mom[2][ci] += dFs * (ey * in.c);
+          S.mom[3][ci] += double(TS(dH));
         }
       } else {
-        TS* pf = dst + P * I + oo;
-        TS* ph = dst + P * (kQ + I) + oo;
+        TS* pf = dst + P * I + o;
+        TS* ph = dst + P * (kQ + I) + o;
         *pf = TS(TC(*pf) - TC(0.5) * dF);
         *ph = TS(TC(*ph) - TC(0.5) * dH);
       }
     }
   }
-  if (MOM && fluid) {
-    in.dmom[oo] = S.mom[0][t];
-    in.dmom[P + oo] = S.mom[1][t];
-    in.dmom[2 * P + oo] = S.mom[2][t];
-    in.dmom[3 * P + oo] = S.mom[3][t];
+  if (MOM) {
+#pragma unroll
+    for (int q = 0; q < Sh::Cells; q++) {
+      if (!fluid[q]) continue;
+      const int ci = t + q * Sh::NT;
+      const long long o = oo[q];
+      in.dmom[o] = S.mom[0][ci];
+      in.dmom[P + o] = S.mom[1][ci];
+      in.dmom[2 * P + o] = S.mom[2][ci];
+      in.dmom[3 * P + o] = S.mom[3][ci];
+    }
   }
 }
 
-template <typename TS, typename TC, int MODE, bool MOM>
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                           int step, Region& R, FluxStage<TC>& S) {
+                                           int step, Region<Sh>& R, FluxStage<Sh, TC, MOM>& S) {
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
     __syncthreads();
-    setup_region(R, in, tr.tx * kTX, tr.ty * kTY);
+    setup_region(R, in, tr.tx * Sh::TX, tr.ty * Sh::TY);
     __syncthreads();
     flux_tile<TS, TC, MODE, MOM>(S, R, in, tr, par);
   }
 }
 
-// FP32 storage: the compiler's own register allocation (up to 7 waves per
-// SIMD). FP64: capped at 80 registers for 6 waves per SIMD, which the
-// moment-summing stage-2 variant would otherwise miss (83 registers, 5
-// waves; 4096^2: 4.18 -> 3.88 ms per launch).
-template <typename TS, typename TC, int MODE, bool MOM = false>
-__global__ __launch_bounds__(kNT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                              const int* stepp) {
-  __shared__ Region R;
-  __shared__ FluxStage<TC> S;
+// FP32 storage: the compiler's own register allocation. FP64: capped at 80
+// registers for 6 waves per SIMD, which the moment-summing stage-2 variant
+// would otherwise miss (83 registers, 5 waves; 4096^2: 4.18 -> 3.88 ms per
+// launch).
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+__global__ __launch_bounds__(Sh::NT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                                 const int* stepp) {
+  __shared__ Region<Sh> R;
+  __shared__ FluxStage<Sh, TC, MOM> S;
   flux_tiles<TS, TC, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S);
 }
 
-template <int MODE, bool MOM = false>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6))) void k_flux64(const InstDev* insts,
-                                                                                       const TileRef* tiles,
-                                                                                       int ntiles, int par,
-                                                                                       const int* stepp) {
-  __shared__ Region R;
-  __shared__ FluxStage<double> S;
+template <int MODE, bool MOM, typename Sh>
+__global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(6))) void k_flux64(
+    const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
+  __shared__ Region<Sh> R;
+  __shared__ FluxStage<Sh, double, MOM> S;
   flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S);
 }
 
-// The flux kernel of a storage/compute type pair.
-template <typename TS, typename TC, int MODE, bool MOM = false>
+// The flux kernel of a storage/compute type pair and tile shape.
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 struct FluxKernel {
-  static constexpr auto fn = &k_flux<TS, TC, MODE, MOM>;
+  static constexpr auto fn = &k_flux<TS, TC, MODE, MOM, Sh>;
 };
-template <int MODE, bool MOM>
-struct FluxKernel<double, double, MODE, MOM> {
-  static constexpr auto fn = &k_flux64<MODE, MOM>;
+template <int MODE, bool MOM, typename Sh>
+struct FluxKernel<double, double, MODE, MOM, Sh> {
+  static constexpr auto fn = &k_flux64<MODE, MOM, Sh>;
 };
 
 // ---------------------------------------------------------------------------
@@ -1058,17 +1096,22 @@ int totals_scratch_doubles() { return 4 * kRedBlocks + 4; }
 // ---------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------
-int tile_w() { return kTX; }
-int tile_h() { return kTY; }
+void flux_tile_dims(int shape, int* w, int* h) {
+  switch (shape) {
+    case 1: *w = Shape1::TX; *h = Shape1::TY; break;
+    case 2: *w = Shape2::TX; *h = Shape2::TY; break;
+    default: *w = Shape0::TX; *h = Shape0::TY; break;
+  }
+}
 int tile_halo() { return kH; }
 int dflux_planes() { return kNDF; }
 
-static int grid_for(const void* kernel) {
+static int grid_for(const void* kernel, int threads) {
   int dev = 0, cus = 256, per = 4;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kNT, 0) != hipSuccess || per < 1) per = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
   long long g = (long long)cus * per;
   g -= g % 8;
   return (int)(g < 8 ? 8 : g);
@@ -1076,35 +1119,42 @@ static int grid_for(const void* kernel) {
 
 static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 
-// Persistent grid sizes of every step kernel of one storage/compute type,
-// computed once (before any stream capture).
-template <typename TS, typename TC>
+// Persistent grid sizes of every step kernel of one storage/compute type and
+// flux tile shape, computed once (before any stream capture).
+template <typename TS, typename TC, typename Sh>
 struct StepGrids {
   int s12, flux2, s3, flux3, s1, flux0, fluxe, fin0, fin2, fin3;
   static const StepGrids& get() {
-    static const StepGrids g = {grid_for((const void*)k_s12<TS, TC>),
-                                grid_for((const void*)FluxKernel<TS, TC, 0, true>::fn),
-                                grid_for((const void*)k_s3<TS, TC>),
-                                grid_for((const void*)FluxKernel<TS, TC, 1>::fn),
-                                grid_for((const void*)k_s1<TS, TC>),
-                                grid_for((const void*)FluxKernel<TS, TC, 0>::fn),
-                                grid_for((const void*)FluxKernel<TS, TC, 2>::fn),
-                                grid_for((const void*)k_fin<TS, TC, 0>),
-                                grid_for((const void*)k_fin<TS, TC, 2>),
-                                grid_for((const void*)k_fin<TS, TC, 3>)};
+    static const StepGrids g = {grid_for((const void*)k_s12<TS, TC>, kNT),
+                                grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT),
+                                grid_for((const void*)k_s3<TS, TC>, kNT),
+                                grid_for((const void*)FluxKernel<TS, TC, 1, false, Sh>::fn, Sh::NT),
+                                grid_for((const void*)k_s1<TS, TC>, kNT),
+                                grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT),
+                                grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT),
+                                grid_for((const void*)k_fin<TS, TC, 0>, kNT),
+                                grid_for((const void*)k_fin<TS, TC, 2>, kNT),
+                                grid_for((const void*)k_fin<TS, TC, 3>, kNT)};
     return g;
   }
 };
 
-void prepare_launch(int fp64) {
-  if (fp64) (void)StepGrids<double, double>::get();
-  else (void)StepGrids<float, float>::get();
+template <typename TS, typename TC>
+static void prepare_shapes() {
+  (void)StepGrids<TS, TC, Shape0>::get();
+  (void)StepGrids<TS, TC, Shape1>::get();
+  (void)StepGrids<TS, TC, Shape2>::get();
 }
 
-template <typename TS, typename TC>
+void prepare_launch(int fp64) {
+  if (fp64) prepare_shapes<double, double>();
+  else prepare_shapes<float, float>();
+}
+
+template <typename TS, typename TC, typename Sh>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
-  const StepGrids<TS, TC>& G = StepGrids<TS, TC>::get();
+  const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
   const int g0 = G.s12, g1 = G.flux2, g2 = G.s3, g3 = G.flux3;
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
@@ -1117,7 +1167,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  FluxKernel<TS, TC, 0, true>::fn<<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, 0, true, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   k_s3<TS, TC><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
@@ -1128,16 +1178,16 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(4, s);
-  FluxKernel<TS, TC, 1>::fn<<<blocks_for(n, g3), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, 1, false, Sh>::fn<<<blocks_for(n, g3), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(4, s);
 }
 
 // RKMETHOD 0, 2, 3: one flux stage.
-template <typename TS, typename TC, int METHOD>
+template <typename TS, typename TC, int METHOD, typename Sh>
 static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
   constexpr int FM = METHOD == 0 ? 0 : 2;
-  const StepGrids<TS, TC>& G = StepGrids<TS, TC>::get();
+  const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
   const int g0 = G.s1, g1 = FM == 0 ? G.flux0 : G.fluxe;
   const int g2 = METHOD == 0 ? G.fin0 : (METHOD == 2 ? G.fin2 : G.fin3);
   const int n = L.ntiles;
@@ -1153,11 +1203,21 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  FluxKernel<TS, TC, FM>::fn<<<blocks_for(n, g1), kNT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, FM, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   k_fin<TS, TC, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
+}
+
+template <typename TS, typename TC, typename Sh>
+static void launch_step_shape(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
+  switch (L.rk_method) {
+    case 0: launch_one_stage<TS, TC, 0, Sh>(L, s, par, tm); break;
+    case 2: launch_one_stage<TS, TC, 2, Sh>(L, s, par, tm); break;
+    case 3: launch_one_stage<TS, TC, 3, Sh>(L, s, par, tm); break;
+    default: launch_rk3<TS, TC, Sh>(L, s, par, tm); break;
+  }
 }
 
 // Advances the step counter of a launch sequence: the kernels of step k read
@@ -1168,11 +1228,10 @@ __global__ void k_tick(int* ctr) {
 
 template <typename TS, typename TC>
 void launch_step(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
-  switch (L.rk_method) {
-    case 0: launch_one_stage<TS, TC, 0>(L, s, par, tm); break;
-    case 2: launch_one_stage<TS, TC, 2>(L, s, par, tm); break;
-    case 3: launch_one_stage<TS, TC, 3>(L, s, par, tm); break;
-    default: launch_rk3<TS, TC>(L, s, par, tm); break;
+  switch (L.flux_shape) {
+    case 1: launch_step_shape<TS, TC, Shape1>(L, s, par, tm); break;
+    case 2: launch_step_shape<TS, TC, Shape2>(L, s, par, tm); break;
+    default: launch_step_shape<TS, TC, Shape0>(L, s, par, tm); break;
   }
   k_tick<<<1, 64, 0, s>>>(L.step_ctr);
 }

@@ -23,12 +23,13 @@ struct StepLaunch {
   int nptiles = 0;
   int rk_method = 1;                // RKMETHOD shared by every instance of the group
   int* step_ctr = nullptr;          // device step counter: step k of a sequence reads k + 1
+  int flux_shape = 0;               // flux tile shape (flux_tile_dims); the tile list is built for it
 };
 
 inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
   return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
          a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
-         a.step_ctr == b.step_ctr;
+         a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape;
 }
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
@@ -77,8 +78,10 @@ class KernelTimer {
   long long count_[kSlots] = {};
 };
 
-int tile_w();
-int tile_h();
+// Flux tile shapes: 0 = 15 x 16 own cells (256 threads), 1 = 31 x 15 (512),
+// 2 = 15 x 32 (512).
+constexpr int kFluxShapes = 3;
+void flux_tile_dims(int shape, int* w, int* h);
 int tile_halo();
 int dflux_planes();
 constexpr int kPointW = 64, kPointH = 4;  // pointwise tile: 64 x 4 cells, 256 threads

@@ -307,8 +307,6 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.flags = (s.p.periodic_x > 0 ? kPeriodicX : 0) | (s.p.periodic_y > 0 ? kPeriodicY : 0) |
             (s.p.mirror_north > 0 ? kMirrorN : 0) | (s.p.mirror_south > 0 ? kMirrorS : 0) |
             (s.p.mirror_east > 0 ? kMirrorE : 0) | (s.p.mirror_west > 0 ? kMirrorW : 0);
-  d.tiles_x = (s.nx + tile_w() - 1) / tile_w();
-  d.tiles_y = (s.ny + tile_h() - 1) / tile_h();
   d.flux_method = s.p.flux_method;
   d.parity0 = s.cur;
   d.step_base = s.step;
@@ -344,11 +342,13 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.Pr_m1 = d.Pr - 1.0;
 }
 
-static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vector<WallRef>& walls,
-                        std::vector<TileRef>& ptiles, int inst) {
-  for (int ty = 0; ty < (s.ny + kPointH - 1) / kPointH; ty++)
-    for (int tx = 0; tx < (s.nx + kPointW - 1) / kPointW; tx++) ptiles.push_back(TileRef{inst, tx, ty, 0});
-  const int TW = tile_w(), TH = tile_h(), H = tile_halo();
+// Flux tiles of one shape with the per-tile solid flag (the tile's region
+// holds a solid cell), instance index 0.
+static std::vector<TileRef> build_flux_tiles(const Solver& s, int shape) {
+  std::vector<TileRef> tiles;
+  int TW = 0, TH = 0;
+  flux_tile_dims(shape, &TW, &TH);
+  const int H = tile_halo();
   const int tx_n = (s.nx + TW - 1) / TW, ty_n = (s.ny + TH - 1) / TH;
   // prefix count of solid cells for the per-tile solid flag
   std::vector<int> pre((size_t)(s.nx + 1) * (s.ny + 1), 0);
@@ -372,8 +372,21 @@ static void build_lists(const Solver& s, std::vector<TileRef>& tiles, std::vecto
       // on the far side: be conservative on edge tiles
       const bool edge = x0 - H < 0 || y0 - H < 0 || x0 + TW + H > s.nx || y0 + TH + H > s.ny;
       if (edge && any_edge_solid) solid = true;
-      tiles.push_back(TileRef{inst, tx, ty, solid ? kTileSolid : 0});
+      tiles.push_back(TileRef{0, tx, ty, solid ? kTileSolid : 0});
     }
+  return tiles;
+}
+
+const std::vector<TileRef>& uqlbm::solver_tiles(Solver& s, int shape) {
+  if (shape < 0 || shape >= kFluxShapes) shape = 0;
+  if (s.tiles_shape[shape].empty()) s.tiles_shape[shape] = build_flux_tiles(s, shape);
+  return s.tiles_shape[shape];
+}
+
+// Pointwise tiles and wall cells (instance index `inst`).
+static void build_lists(const Solver& s, std::vector<WallRef>& walls, std::vector<TileRef>& ptiles, int inst) {
+  for (int ty = 0; ty < (s.ny + kPointH - 1) / kPointH; ty++)
+    for (int tx = 0; tx < (s.nx + kPointW - 1) / kPointW; tx++) ptiles.push_back(TileRef{inst, tx, ty, 0});
   for (int y = 0; y < s.ny; y++)
     for (int x = 0; x < s.nx; x++) {
       if (type_at(s, x, y) != 2) continue;
@@ -493,10 +506,11 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   for (void* b : {s->state[0], s->state[1], s->f2, s->f3}) HIP_TRY(hipMemset(b, 0, plane * kPlanes * es));
   HIP_TRY(hipMemset(s->dflux, 0, plane * dflux_planes() * es));
 
-  std::vector<TileRef> tiles;
   std::vector<WallRef> walls;
   std::vector<TileRef> ptiles;
-  build_lists(*s, tiles, walls, ptiles, 0);
+  build_lists(*s, walls, ptiles, 0);
+  s->flux_shape = default_flux_shape(precision);
+  const std::vector<TileRef>& tiles = solver_tiles(*s, s->flux_shape);
   s->ntiles = (int)tiles.size();
   s->nwalls = (int)walls.size();
   s->nptiles = (int)ptiles.size();
@@ -508,7 +522,6 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
     HIP_TRY(hipMalloc((void**)&s->walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(s->walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
-  s->tiles_host = std::move(tiles);
   s->walls_host = std::move(walls);
   s->ptiles_host = std::move(ptiles);
   st = upload_inst(*s);
@@ -566,6 +579,12 @@ static void launch_steps(const StepLaunch& L, int precision, hipStream_t stream,
 
 // HIP graph replay of the step sequence (on unless lbm_set_graphs(0)).
 static bool g_graphs_enabled = true;
+
+// Flux tile shape used by solvers, batches and servers created from now on,
+// per precision: set by lbm_set_flux_shape or by the autotuner.
+static int g_flux_shape[2] = {0, 0};
+
+int uqlbm::default_flux_shape(int precision) { return g_flux_shape[precision == LBM_PREC_FP64 ? 0 : 1]; }
 
 static int capture_steps(const StepLaunch& L, int precision, hipStream_t stream, int nsteps,
                          hipGraphExec_t* out) {
@@ -684,8 +703,8 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
   if (s.failed) return LBM_FAILED;
   if (nsteps == 0) return LBM_OK;
-  StepLaunch L{s.inst_dev, s.tiles_dev, s.ntiles,   s.walls_dev, s.nwalls,
-               s.ptiles_dev, s.nptiles, s.p.rk_method, s.step_ctr};
+  StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
+               s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   std::vector<Solver*> one{&s};
   int st = LBM_OK;
   if (want_residual && nsteps > 1) {
@@ -867,15 +886,28 @@ extern "C" int lbm_batch_create(lbm_solver* const* members, int32_t n, lbm_batch
   return batch_finish(*b, out);
 }
 
-int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
+// Uploads the batch's flux tile list for `shape` (members' tiles with their
+// batch index).
+int uqlbm::batch_set_shape(Batch& b, int shape) {
   std::vector<TileRef> tiles;
-  std::vector<WallRef> walls;
-  std::vector<TileRef> ptiles;
-  for (size_t k = 0; k < b.members.size(); k++) {
-    for (TileRef t : b.members[k]->tiles_host) {
+  for (size_t k = 0; k < b.members.size(); k++)
+    for (TileRef t : solver_tiles(*b.members[k], shape)) {
       t.inst = (int)k;
       tiles.push_back(t);
     }
+  if (b.tiles_dev) HIP_TRY(hipFree(b.tiles_dev));
+  b.tiles_dev = nullptr;
+  HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
+  HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  b.ntiles = (int)tiles.size();
+  b.flux_shape = shape;
+  return LBM_OK;
+}
+
+int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
+  std::vector<WallRef> walls;
+  std::vector<TileRef> ptiles;
+  for (size_t k = 0; k < b.members.size(); k++) {
     for (TileRef t : b.members[k]->ptiles_host) {
       t.inst = (int)k;
       ptiles.push_back(t);
@@ -885,14 +917,13 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
       walls.push_back(w);
     }
   }
-  b.ntiles = (int)tiles.size();
   b.nwalls = (int)walls.size();
   b.nptiles = (int)ptiles.size();
   HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&b.step_ctr, sizeof(int)));
-  HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
-  HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  int st = batch_set_shape(b, default_flux_shape(b.precision));
+  if (st != LBM_OK) return st;
   if (!walls.empty()) {
     HIP_TRY(hipMalloc((void**)&b.walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(b.walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
@@ -911,8 +942,8 @@ uqlbm::Batch::~Batch() {
 }
 
 int uqlbm::batch_step(Batch& b, int nsteps) {
-  StepLaunch L{b.inst_dev, b.tiles_dev, b.ntiles,  b.walls_dev, b.nwalls,
-               b.ptiles_dev, b.nptiles, b.rk_method, b.step_ctr};
+  StepLaunch L{b.inst_dev,    b.tiles_dev, b.ntiles,    b.walls_dev, b.nwalls,
+               b.ptiles_dev, b.nptiles,   b.rk_method, b.step_ctr,  b.flux_shape};
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get());
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);
@@ -929,4 +960,93 @@ extern "C" int lbm_batch_step(lbm_batch* hb, int32_t nsteps) {
 }
 
 extern "C" void lbm_batch_destroy(lbm_batch* b) { delete reinterpret_cast<Batch*>(b); }
+
+extern "C" int lbm_set_flux_shape(int32_t precision, int32_t shape) {
+  if (precision != LBM_PREC_FP64 && precision != LBM_PREC_MIXED) return fail_with(LBM_ERR_ARG, "unknown precision");
+  if (shape < 0 || shape >= kFluxShapes) return fail_with(LBM_ERR_ARG, "flux tile shape %d out of range", shape);
+  g_flux_shape[precision == LBM_PREC_FP64 ? 0 : 1] = shape;
+  return LBM_OK;
+}
+
+extern "C" int lbm_flux_shape_dims(int32_t shape, int32_t* w, int32_t* h) {
+  if (shape < 0 || shape >= kFluxShapes || !w || !h) return fail_with(LBM_ERR_ARG, "bad flux shape query");
+  int ww = 0, hh = 0;
+  flux_tile_dims(shape, &ww, &hh);
+  *w = ww;
+  *h = hh;
+  return LBM_OK;
+}
+
+// Tile-shape autotuner: times `steps` steps of the batch with each flux tile
+// shape (after one untimed step that also captures the step graphs), keeps
+// the fastest, and restores every member's state, step count and failure
+// flag, so tuning leaves the solution untouched.
+extern "C" int lbm_batch_autotune(lbm_batch* hb, int32_t steps, int32_t* shape_out, double* ms_per_step_out) {
+  if (!hb || steps < 1) return fail_with(LBM_ERR_ARG, "bad autotune arguments");
+  Batch& b = *reinterpret_cast<Batch*>(hb);
+  struct Saved {
+    void* buf = nullptr;
+    int cur = 0, step = 0, fail = INT_MAX;
+    bool failed = false;
+  };
+  std::vector<Saved> saved(b.members.size());
+  auto release = [&]() {
+    for (auto& v : saved)
+      if (v.buf) (void)hipFree(v.buf);
+  };
+  const size_t es = b.precision == LBM_PREC_FP64 ? sizeof(double) : sizeof(float);
+  for (size_t k = 0; k < b.members.size(); k++) {
+    Solver& s = *b.members[k];
+    const size_t bytes = (size_t)s.pitch * s.ny * kPlanes * es;
+    if (hipMalloc(&saved[k].buf, bytes) != hipSuccess) {
+      release();
+      return fail_with(LBM_ERR_NOMEM, "autotune: no room for a state snapshot");
+    }
+    HIP_TRY(hipMemcpyAsync(saved[k].buf, s.state[s.cur], bytes, hipMemcpyDeviceToDevice, b.stream));
+    HIP_TRY(hipMemcpyAsync(&saved[k].fail, s.fail, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+    saved[k].cur = s.cur;
+    saved[k].step = s.step;
+    saved[k].failed = s.failed;
+  }
+  HIP_TRY(hipStreamSynchronize(b.stream));
+  double best = 0;
+  int best_shape = b.flux_shape;
+  for (int shape = 0; shape < kFluxShapes; shape++) {
+    int st = batch_set_shape(b, shape);
+    if (st == LBM_OK) st = batch_step(b, 1);
+    if (st == LBM_OK) st = batch_step(b, steps);
+    if (st < 0) {
+      release();
+      return st;
+    }
+    const double ms = g_last_timing.total_ms / steps;
+    if (ms_per_step_out) ms_per_step_out[shape] = ms;
+    if (shape == 0 || ms < best) {
+      best = ms;
+      best_shape = shape;
+    }
+  }
+  for (size_t k = 0; k < b.members.size(); k++) {
+    Solver& s = *b.members[k];
+    const size_t bytes = (size_t)s.pitch * s.ny * kPlanes * es;
+    s.cur = saved[k].cur;
+    s.step = saved[k].step;
+    s.failed = saved[k].failed;
+    HIP_TRY(hipMemcpyAsync(s.state[s.cur], saved[k].buf, bytes, hipMemcpyDeviceToDevice, b.stream));
+    HIP_TRY(hipMemcpyAsync(s.fail, &saved[k].fail, sizeof(int), hipMemcpyHostToDevice, b.stream));
+    HIP_TRY(hipStreamSynchronize(b.stream));
+    const int st = upload_inst(s);
+    if (st != LBM_OK) {
+      release();
+      return st;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(b.stream));
+  release();
+  const int st = batch_set_shape(b, best_shape);
+  if (st != LBM_OK) return st;
+  g_flux_shape[b.precision == LBM_PREC_FP64 ? 0 : 1] = best_shape;
+  if (shape_out) *shape_out = best_shape;
+  return LBM_OK;
+}
 

@@ -49,7 +49,7 @@ struct Solver {
 
   std::vector<int> ids_host;        // [y][nx]
   std::vector<uint8_t> ctype_host;  // [y][nx]
-  std::vector<TileRef> tiles_host;
+  std::vector<TileRef> tiles_shape[kFluxShapes];  // flux tiles per shape, built on first use
   std::vector<WallRef> walls_host;
   std::vector<TileRef> ptiles_host;
 
@@ -70,6 +70,7 @@ struct Solver {
   TileRef* ptiles_dev = nullptr;
   double* macro_dev = nullptr;
   int* step_ctr = nullptr;          // device step counter of this solver's launches
+  int flux_shape = 0;               // flux tile shape of tiles_dev
   double* totals_dev = nullptr;     // conserved-totals reduction scratch (allocated on first use)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
@@ -95,6 +96,7 @@ struct Batch {
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
   int* step_ctr = nullptr;
+  int flux_shape = 0;
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   Batch() = default;
@@ -109,6 +111,7 @@ struct Batch {
     walls_dev = o.walls_dev;
     ptiles_dev = o.ptiles_dev;
     step_ctr = o.step_ctr;
+    flux_shape = o.flux_shape;
     graphs = std::move(o.graphs);
     ntiles = o.ntiles;
     nwalls = o.nwalls;
@@ -125,6 +128,9 @@ struct Batch {
 };
 
 int fail_with(int status, const char* fmt, ...);
+const std::vector<TileRef>& solver_tiles(Solver& s, int shape);
+int default_flux_shape(int precision);
+int batch_set_shape(Batch& b, int shape);
 size_t solver_footprint(int nx, int ny, int precision);
 int upload_inst(Solver& s);
 int batch_finish(Batch& b, lbm_batch** out);

@@ -66,6 +66,7 @@ struct Server {
   int ntiles = 0, nwalls = 0, nptiles = 0;
   bool lists_dirty = true;
   int* step_ctr = nullptr;
+  int flux_shape = 0;
   GraphCache graphs;  // step graphs of the current active set (recaptured when it changes)
   ~Server();
 };
@@ -92,11 +93,12 @@ Server::~Server() {
   } while (0)
 
 static int rebuild_lists(Server& sv) {
+  sv.flux_shape = default_flux_shape(sv.precision);
   std::vector<TileRef> tiles;
   std::vector<WallRef> walls;
   std::vector<TileRef> ptiles;
   for (size_t k = 0; k < sv.active.size(); k++) {
-    for (TileRef t : sv.active[k].solver->tiles_host) {
+    for (TileRef t : solver_tiles(*sv.active[k].solver, sv.flux_shape)) {
       t.inst = (int)k;
       tiles.push_back(t);
     }
@@ -286,8 +288,8 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     std::vector<Solver*> members;
     members.reserve(sv.active.size());
     for (auto& a : sv.active) members.push_back(a.solver);
-    StepLaunch L{sv.table_dev, sv.tiles_dev, sv.ntiles,   sv.walls_dev, sv.nwalls,
-                 sv.ptiles_dev, sv.nptiles, sv.rk_method, sv.step_ctr};
+    StepLaunch L{sv.table_dev,  sv.tiles_dev, sv.ntiles,     sv.walls_dev, sv.nwalls,
+                 sv.ptiles_dev, sv.nptiles,   sv.rk_method, sv.step_ctr,  sv.flux_shape};
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
     st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round, &sv.graphs);

@@ -60,6 +60,9 @@ def lib():
     L.lbm_set_kernel_timing.argtypes = [i32]
     L.lbm_set_graphs.argtypes = [i32]
     L.lbm_get_totals.argtypes = [vp, P(d)]
+    L.lbm_set_flux_shape.argtypes = [i32, i32]
+    L.lbm_flux_shape_dims.argtypes = [i32, P(i32), P(i32)]
+    L.lbm_batch_autotune.argtypes = [vp, i32, P(i32), P(d)]
     L.lbm_get_timing.argtypes = [P(LbmTiming)]
     L.lbm_clock_s.restype = d
     if hasattr(L, "lbm_server_create"):
@@ -88,6 +91,21 @@ def _check(status: int, where: str, allow_failed: bool = False) -> int:
 
 def set_kernel_timing(enable: bool) -> None:
     lib().lbm_set_kernel_timing(1 if enable else 0)
+
+
+FLUX_SHAPES = 3
+
+
+def set_flux_shape(precision: str, shape: int) -> None:
+    """Flux tile shape of solvers, batches and servers of `precision`
+    created from now on (0: 15x16, 1: 31x15, 2: 15x32 own cells)."""
+    _check(lib().lbm_set_flux_shape(PRECISIONS[precision], int(shape)), "lbm_set_flux_shape")
+
+
+def flux_shape_dims(shape: int) -> tuple[int, int]:
+    w, h = C.c_int32(0), C.c_int32(0)
+    _check(lib().lbm_flux_shape_dims(int(shape), C.byref(w), C.byref(h)), "lbm_flux_shape_dims")
+    return w.value, h.value
 
 
 def set_graphs(enable: bool) -> None:
@@ -187,6 +205,14 @@ class Batch:
 
     def step(self, n: int) -> int:
         return _check(lib().lbm_batch_step(self._h, int(n)), "lbm_batch_step", allow_failed=True)
+
+    def autotune(self, steps: int = 5) -> tuple[int, list[float]]:
+        """Pick the fastest flux tile shape for this batch (state unchanged);
+        returns the shape and each shape's milliseconds per step."""
+        shape = C.c_int32(0)
+        ms = np.zeros(FLUX_SHAPES)
+        _check(lib().lbm_batch_autotune(self._h, int(steps), C.byref(shape), dptr(ms)), "lbm_batch_autotune")
+        return shape.value, ms.tolist()
 
     def close(self):
         if self._h:
