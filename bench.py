@@ -72,11 +72,17 @@ def measured_traffic(kernel: str, workload: str):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
-def batched_leg(members, precision: str, steps: int, warmup: int, timing_steps: int):
-    """Warm up, time exactly `steps` steps of the whole batch by the host
-    clock, then take a short per-kernel timing sample (HIP events)."""
+def batched_leg(members, precision: str, steps: int, warmup: int, timing_steps: int, tune_steps: int):
+    """Pick the flux tile shape with the autotuner (state unchanged), warm
+    up, time exactly `steps` steps of the whole batch by the host clock,
+    then take a short per-kernel timing sample (HIP events)."""
     solvers = [lbm.Solver(c, precision) for c in members]
     b = lbm.Batch(solvers)
+    tune = None
+    if tune_steps:
+        shape, ms = b.autotune(tune_steps)
+        tune = {"flux_tile_shape": shape, "tile_dims": list(lbm.flux_shape_dims(shape)),
+                "ms_per_step_by_shape": [round(m, 3) for m in ms]}
     if warmup:
         b.step(warmup)
     t0 = time.perf_counter()
@@ -86,7 +92,7 @@ def batched_leg(members, precision: str, steps: int, warmup: int, timing_steps: 
     b.step(timing_steps)
     kt = lbm.last_timing()["kernels"]
     lbm.set_kernel_timing(False)
-    return solvers, b, st, sec, kt
+    return solvers, b, st, sec, kt, tune
 
 
 def roofline(ktimes: dict, cells: int, es: int, workload: str) -> dict:
@@ -174,6 +180,10 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--timing-steps", type=int, default=10, help="steps of the per-kernel timing sample")
+    ap.add_argument("--tune-steps", type=int, default=3,
+                    help="steps per tile shape of the autotuner before the timed run (0: no tuning)")
+    ap.add_argument("--flux-shape", type=int, default=-1,
+                    help="force this flux tile shape instead of tuning (profiling runs)")
     ap.add_argument("--plain-steps", type=int, default=20, help="step sample of the plain path")
     ap.add_argument("--riemann-n", type=int, default=4096)
     ap.add_argument("--riemann-steps", type=int, default=20)
@@ -215,6 +225,10 @@ def main() -> int:
         args.serve_seconds, args.cpu_seconds, args.accuracy_steps = 0.5, 3.0, 20
     if lbm.device_count() < 1:
         raise SystemExit("no gfx950 device visible")
+    if args.flux_shape >= 0:
+        args.tune_steps = 0
+        for prec in ("fp64", "mixed"):
+            lbm.set_flux_shape(prec, args.flux_shape)
 
     legs = {"headline", "mixed", "plain", "riemann", "serve", "cpu"} if args.legs == "all" else \
         set(args.legs.split(","))
@@ -227,9 +241,10 @@ def main() -> int:
 
     # --- headline: batched FP64 ------------------------------------------
     if "headline" in legs:
-        solvers, batch, st, sec, kt = batched_leg(members, "fp64", args.steps, args.warmup, args.timing_steps)
+        solvers, batch, st, sec, kt, tune = batched_leg(members, "fp64", args.steps, args.warmup,
+                                                        args.timing_steps, args.tune_steps)
     else:  # profiling runs of other legs
-        solvers, batch, st, sec, kt = [], None, None, float("nan"), {}
+        solvers, batch, st, sec, kt, tune = [], None, None, float("nan"), {}, None
     mlups = cells * args.steps / sec / 1e6
     log(f"batched fp64: {sec:.2f} s for {args.steps} steps -> {mlups:.1f} MLUPS (status {st})")
     result = {
@@ -255,6 +270,7 @@ def main() -> int:
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kt.items()},
         "step_roofline_frac": round(mlups * 1040.0 / 1e3 / HBM_PEAK_GBS, 4),
         "status": st,
+        "autotune": tune,
     }
     if batch is not None:
         batch.close()
@@ -264,15 +280,15 @@ def main() -> int:
     mlups_m = None
     if "mixed" in legs:
         # --- batched mixed precision and its accuracy against FP64 ----------
-        solvers, batch, st_m, sec_m, kt_m = batched_leg(members, "mixed", args.steps, args.warmup,
-                                                        args.timing_steps)
+        solvers, batch, st_m, sec_m, kt_m, tune_m = batched_leg(members, "mixed", args.steps, args.warmup,
+                                                                args.timing_steps, args.tune_steps)
         mlups_m = cells * args.steps / sec_m / 1e6
         batch.close()
         for s in solvers:
             s.close()
         acc = accuracy(members, args.accuracy_steps, args.accuracy_instances)
         result["batched_mixed"] = {
-            "mlups": round(mlups_m, 2), "status": st_m,
+            "mlups": round(mlups_m, 2), "status": st_m, "autotune": tune_m,
             "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kt_m.items()},
             "rel_l2_vs_fp64": acc,
             "meets_1e-6": bool(max(acc["rho"], acc["u"], acc["T"]) <= 1e-6),
