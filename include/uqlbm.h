@@ -149,6 +149,14 @@ int lbm_get_dist(lbm_solver* s, double* f, double* h);
  * used for checkpoint/resume and for injecting states in tests. */
 int lbm_set_dist(lbm_solver* s, const double* f, const double* h);
 
+/* Temperature-dependent viscosity of the reference's OpenCL front end
+ * (CODE/Python/simLB/codeCL.py:59-69, 987-990), replacing the deck's
+ * constant mu in the relaxation times tf = mu(T)/p of every later step:
+ * model 0 constant (deck mu), 1 VHS mu_ref (T/T_ref)^(0.5 + param),
+ * 2 Sutherland mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S) with S = param.
+ * T is the cell temperature in deck units. */
+int lbm_set_viscosity(lbm_solver* s, int32_t model, double mu_ref, double T_ref, double param);
+
 /* Conserved totals over the fluid cells (cellType 0), FP64 sums:
  * out[0] = sum f (mass), out[1] = sum f e_x, out[2] = sum f e_y (momentum),
  * out[3] = sum h (total energy), e in physical units (lattice x c). The

@@ -62,6 +62,12 @@ struct orc {
   double *dF2, *dH2;                          /* [x][y][i] */
 
   double step_seconds;
+
+  /* viscosity model of the OpenCL front end (PY/codeCL.py:59-69, 987-990):
+   * 0 constant (deck mu), 1 VHS mu_ref (T/T_ref)^(0.5+param),
+   * 2 Sutherland mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S) with S = param */
+  int visc_model;
+  double visc_mu_ref, visc_T_ref, visc_param;
 };
 
 #define CELL(o, x, y) ((size_t)(x) * (size_t)(o)->ny + (size_t)(y))
@@ -196,9 +202,19 @@ static void finish_macro(const orc* o, double rho, double rux, double ruy, doubl
   m->p = rho * o->p.R * T;
 }
 
-/* relaxation times, CPP/cpuLBM.cpp:952-955 */
+/* viscosity at temperature T (constant for the C++ solver; VHS and
+ * Sutherland laws of PY/codeCL.py:987-990) */
+static double viscosity(const orc* o, double T) {
+  if (o->visc_model == 1) return o->visc_mu_ref * pow(T / o->visc_T_ref, 0.5 + o->visc_param);
+  if (o->visc_model == 2)
+    return o->visc_mu_ref * pow(T / o->visc_T_ref, 3.0 / 2.0) *
+           ((o->visc_T_ref + o->visc_param) / (T + o->visc_param));
+  return o->p.mu;
+}
+
+/* relaxation times, CPP/cpuLBM.cpp:952-955 (PY/codeCL.py:992-994) */
 static void finish_tau(const orc* o, macro_t* m) {
-  m->tf = o->p.mu / m->p;
+  m->tf = viscosity(o, m->T) / m->p;
   m->th = m->tf / o->p.Pr;
   m->thf = (m->th * m->tf) / (m->tf - m->th);
 }
@@ -927,3 +943,13 @@ double orc_cflux(const orc* o, const double* flux, int32_t x, int32_t y, int32_t
 }
 
 double orc_step_seconds(const orc* o) { return o->step_seconds; }
+
+int orc_set_viscosity(orc* o, int32_t model, double mu_ref, double T_ref, double param) {
+  if (model < 0 || model > 2) return -1;
+  if (model > 0 && !(mu_ref > 0 && T_ref > 0)) return -1;
+  o->visc_model = model;
+  o->visc_mu_ref = mu_ref;
+  o->visc_T_ref = T_ref;
+  o->visc_param = param;
+  return 0;
+}

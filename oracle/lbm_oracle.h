@@ -57,6 +57,10 @@ void orc_boundary_fluxes(orc* o, const double* f, const double* h,
 /* cFLUX: net flux of direction i at (x,y) from a [x][y][i][2] flux field. */
 double orc_cflux(const orc* o, const double* flux, int32_t x, int32_t y,
                  int32_t i);
+/* Viscosity law (PY/codeCL.py:59-69, 987-990): 0 constant deck mu, 1 VHS
+ * mu_ref (T/T_ref)^(0.5+param), 2 Sutherland mu_ref (T/T_ref)^1.5
+ * (T_ref+S)/(T+S), S = param. Returns -1 on bad arguments. */
+int orc_set_viscosity(orc* o, int32_t model, double mu_ref, double T_ref, double param);
 /* Seconds spent inside orc_step (steady clock around the step loop only). */
 double orc_step_seconds(const orc* o);
 

@@ -60,6 +60,7 @@ def lib():
         L.orc_cflux.argtypes = [vp, P(d), i32, i32, i32]
         L.orc_step_seconds.restype = d
         L.orc_step_seconds.argtypes = [vp]
+        L.orc_set_viscosity.argtypes = [vp, i32, d, d, d]
         _lib = L
     return _lib
 
@@ -95,6 +96,16 @@ class Oracle:
                                    self.deck.bnd_ptr(), C.byref(self.deck.props))
         if not self._h:
             raise ValueError("orc_create rejected the deck")
+        v = case.meta.get("viscosity") if isinstance(case.meta, dict) else None
+        if v:
+            self.set_viscosity(v["model"], v["mu_ref"], v["T_ref"], v.get("param", 0.0))
+
+    def set_viscosity(self, model, mu_ref: float, T_ref: float, param: float = 0.0) -> None:
+        """Viscosity law: "constant", "vhs" (exponent 0.5 + param) or
+        "sutherland" (constant S = param)."""
+        code = {"constant": 0, "vhs": 1, "sutherland": 2}[model] if isinstance(model, str) else int(model)
+        if lib().orc_set_viscosity(self._h, code, float(mu_ref), float(T_ref), float(param)) != 0:
+            raise ValueError("bad viscosity law")
 
     def close(self):
         if self._h:

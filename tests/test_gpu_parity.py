@@ -155,3 +155,21 @@ def test_one_stage_steppers(gpu_lib, rk):
         # the equilibrium check trips on both sides at the same step
         c = cases.flat_plate(nx=48, ny=32, plate_start=12, steps=10, rk_method=rk)
         _compare(c, 10, gpu_lib, may_fail=True)
+
+
+@pytest.mark.parametrize("law", ["vhs", "sutherland"])
+def test_temperature_dependent_viscosity(gpu_lib, law):
+    """VHS and Sutherland viscosity (the OpenCL front end's laws,
+    PY/codeCL.py:987-990) on a heated Couette deck, the NND plate and the
+    Riemann problem (SI units, strong temperature contrasts)."""
+    c = cases.couette(40, u_wall=0.3, steps=20)
+    c.meta["viscosity"] = {"model": law, "mu_ref": c.params["mu"], "T_ref": 0.9, "param": 0.24}
+    _compare(c, 20, gpu_lib)
+    p = cases.flat_plate(64, 32, plate_start=16, steps=12)
+    p.meta["viscosity"] = {"model": law, "mu_ref": 1.8e-5, "T_ref": 273.0,
+                           "param": 0.24 if law == "vhs" else 110.4}
+    _compare(p, 12, gpu_lib, conditioned=True)
+    r = cases.riemann(64, steps=8)
+    r.meta["viscosity"] = {"model": law, "mu_ref": 1.8e-5, "T_ref": 273.0,
+                           "param": 0.24 if law == "vhs" else 110.4}
+    _compare(r, 8, gpu_lib, conditioned=True)

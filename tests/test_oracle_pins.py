@@ -332,6 +332,36 @@ def test_pure_relaxation_closed_form():
     np.testing.assert_allclose(ratio, _relaxation_factor(a), rtol=1e-9)
 
 
+@pytest.mark.parametrize("law,param,mu_expect", [
+    # VHS (PY/codeCL.py:987): mu_ref (T/T_ref)^(0.5 + upsilon), T/T_ref = 1.25
+    ("vhs", 0.25, lambda mu_ref: mu_ref * 1.25 ** 0.75),
+    ("vhs", -0.5, lambda mu_ref: mu_ref),
+    # Sutherland (PY/codeCL.py:989): mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S)
+    ("sutherland", 0.4, lambda mu_ref: mu_ref * 1.25 ** 1.5 * (0.8 + 0.4) / (1.0 + 0.4)),
+])
+def test_viscosity_law_sets_relaxation(law, param, mu_expect):
+    """The pure-relaxation state (T stays T0 = 1) relaxes with tau =
+    mu(T0)/p for the temperature-dependent laws of the OpenCL front end:
+    the decay ratio is R(dt/(2 tau)) with mu(T0) from the published law at
+    T0/T_ref = 1.25 (a wrong exponent, an inverted ratio or the deck mu in
+    its place would each miss it)."""
+    c = cases.shear_wave(6, u0=0.0, tau=0.05)
+    mu_ref = 0.03
+    c.meta["viscosity"] = {"model": law, "mu_ref": mu_ref, "T_ref": 0.8, "param": param}
+    o = Oracle(c)
+    f, h = o.get_dist()
+    k = o.consts()
+    p = np.zeros(13)
+    p[9], p[11], p[10], p[12] = 1.0, 1.0, -1.0, -1.0
+    f2 = f + 1e-3 * p
+    o.set_dist(f2, h)
+    o.step(1)
+    f3, _ = o.get_dist()
+    tau = mu_expect(mu_ref) / 1.0      # p = rho R T = 1
+    ratio = (f3 - f)[..., 9] / (f2 - f)[..., 9]
+    np.testing.assert_allclose(ratio, _relaxation_factor(k["dt"] / (2 * tau)), rtol=1e-9)
+
+
 def test_relaxation_factor_second_order():
     """R(a) must match exp(-2a) (exact decay over dt = 2a*tau) to O(a^3)."""
     errs = [abs(_relaxation_factor(a) - math.exp(-2 * a)) for a in (1e-2, 5e-3, 2.5e-3)]

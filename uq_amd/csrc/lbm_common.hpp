@@ -55,6 +55,10 @@ struct InstDev {
   // per-cell arithmetic multiplies instead of divides
   double isRTc, iTc, ibR;  // 1/sqrt(R*Tc), 1/Tc, 1/(b*R)
   double imu, Pr_m1;       // 1/mu, Pr - 1
+  // viscosity law: 0 constant mu; 1 VHS mu_ref (T/T_ref)^vexp; 2 Sutherland
+  // mu_ref (T/T_ref)^1.5 (T_ref + vS)/(T + vS)
+  int visc_model;
+  double mu_ref, T_ref_v, iT_ref_v, vexp, vS;
 };
 
 // One tile of one instance: the unit of work of a block.

@@ -151,6 +151,16 @@ struct Macro {
   double itf, ith, ithf;  // 1/tf, 1/th, 1/thf
 };
 
+// Temperature-dependent viscosity of the OpenCL front end
+// (PY/codeCL.py:987-990): VHS mu_ref (T/T_ref)^vexp or Sutherland
+// mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S). Kept out of line so that the
+// constant-viscosity path carries none of pow's registers.
+__device__ __noinline__ static double viscosity_of(const InstDev& in, double T) {
+  const double r = T * in.iT_ref_v;
+  return in.visc_model == 1 ? in.mu_ref * pow(r, in.vexp)
+                            : in.mu_ref * pow(r, 1.5) * ((in.T_ref_v + in.vS) / (T + in.vS));
+}
+
 __device__ __forceinline__ void finish_macro(const InstDev& in, double rho, double rux, double ruy,
                                              double sh, Macro& m) {
   const double irho = 1.0 / rho;
@@ -160,7 +170,7 @@ __device__ __forceinline__ void finish_macro(const InstDev& in, double rho, doub
   const double usq = m.ux * m.ux + m.uy * m.uy;
   m.T = (2.0 * (sh * irho) - usq) * in.ibR;
   m.p = rho * in.R * m.T;
-  m.itf = m.p * in.imu;
+  m.itf = in.visc_model == 0 ? m.p * in.imu : m.p / viscosity_of(in, m.T);
   m.ith = m.itf * in.Pr;
   m.ithf = m.itf * in.Pr_m1;
 }

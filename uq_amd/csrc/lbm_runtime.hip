@@ -340,6 +340,12 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.ibR = 1.0 / d.bR;
   d.imu = 1.0 / d.mu;
   d.Pr_m1 = d.Pr - 1.0;
+  d.visc_model = s.visc_model;
+  d.mu_ref = s.mu_ref;
+  d.T_ref_v = s.T_ref_v;
+  d.iT_ref_v = s.T_ref_v > 0 ? 1.0 / s.T_ref_v : 0.0;
+  d.vexp = 0.5 + s.visc_param;
+  d.vS = s.visc_param;
 }
 
 // Flux tiles of one shape with the per-tile solid flag (the tile's region
@@ -845,6 +851,20 @@ extern "C" int lbm_get_info(const lbm_solver* h, lbm_info* out) {
 }
 
 extern "C" void lbm_destroy(lbm_solver* h) { delete reinterpret_cast<Solver*>(h); }
+
+extern "C" int lbm_set_viscosity(lbm_solver* h, int32_t model, double mu_ref, double T_ref, double param) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  if (model < 0 || model > 2) return fail_with(LBM_ERR_ARG, "unknown viscosity law %d", model);
+  if (model > 0 && !(mu_ref > 0 && T_ref > 0 && std::isfinite(mu_ref) && std::isfinite(T_ref) && std::isfinite(param)))
+    return fail_with(LBM_ERR_ARG, "viscosity law needs positive mu_ref and T_ref");
+  if (model == 2 && !(param >= 0)) return fail_with(LBM_ERR_ARG, "Sutherland constant must be >= 0");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  s.visc_model = model;
+  s.mu_ref = mu_ref;
+  s.T_ref_v = T_ref;
+  s.visc_param = param;
+  return upload_inst(s);
+}
 
 extern "C" int lbm_set_kernel_timing(int32_t enable) {
   g_timing_enabled = enable != 0;

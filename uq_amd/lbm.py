@@ -61,6 +61,7 @@ def lib():
     L.lbm_set_graphs.argtypes = [i32]
     L.lbm_get_totals.argtypes = [vp, P(d)]
     L.lbm_set_flux_shape.argtypes = [i32, i32]
+    L.lbm_set_viscosity.argtypes = [vp, i32, d, d, d]
     L.lbm_flux_shape_dims.argtypes = [i32, P(i32), P(i32)]
     L.lbm_batch_autotune.argtypes = [vp, i32, P(i32), P(d)]
     L.lbm_get_timing.argtypes = [P(LbmTiming)]
@@ -136,8 +137,18 @@ class Solver:
                "lbm_create")
         self._h = h
         self.precision = precision
+        v = case.meta.get("viscosity") if isinstance(case.meta, dict) else None
+        if v:
+            self.set_viscosity(v["model"], v["mu_ref"], v["T_ref"], v.get("param", 0.0))
         if init:
             self.init()
+
+    def set_viscosity(self, model, mu_ref: float, T_ref: float, param: float = 0.0) -> None:
+        """Viscosity law: "constant", "vhs" (exponent 0.5 + param) or
+        "sutherland" (constant S = param)."""
+        code = {"constant": 0, "vhs": 1, "sutherland": 2}[model] if isinstance(model, str) else int(model)
+        _check(lib().lbm_set_viscosity(self._h, code, float(mu_ref), float(T_ref), float(param)),
+               "lbm_set_viscosity")
 
     @property
     def handle(self):
