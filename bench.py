@@ -297,6 +297,9 @@ def main() -> int:
 
     if "plain" in legs:
         # --- plain one-instance-per-launch FP64 path --------------------------
+        # (the untuned default 15x16 flux tiles: the tuned batch shape, 15x32,
+        # leaves the small instances fewer and larger tiles)
+        lbm.set_flux_shape("fp64", 0)
         plain = [lbm.Solver(c, "fp64") for c in members]
         for s in plain:
             s.step(2)

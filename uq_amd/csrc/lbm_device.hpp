@@ -207,26 +207,28 @@ __device__ __forceinline__ T nnd(T v, T s0, T s1, T s2) {
 }
 
 // Jiang-Shu WENO5 with d = (0.1, 0.6, 0.3), eps = 1e-6 on |e|*S. The
-// normalised weights w_k = a_k / sum(a) with a_k = d_k/(eps+b_k)^2 are formed
-// as A_k / sum(A) with A_k = d_k * prod_{j!=k} (eps+b_j)^2 — the same ratio,
-// one division instead of six — in FP64 for both precisions.
+// smoothness indicators and candidate values are formed on the unscaled S
+// and scaled afterwards (beta(vS) = v^2 beta(S), q(vS) = v q(S)), and the
+// normalised weights w_k = a_k / sum(a) with a_k = d_k/(eps+b_k)^2 are
+// formed as A_k / sum(A) with A_k = d_k * prod_{j!=k} (eps+b_j)^2 — the
+// same ratio, one division instead of six — in FP64 for both precisions.
 template <typename T>
 __device__ __forceinline__ T weno5(T v, T s0, T s1, T s2, T s3, T s4) {
-  const T q0 = s0 * v, q1 = s1 * v, q2 = s2 * v, q3 = s3 * v, q4 = s4 * v;
-  const T c0 = q0 - T(2) * q1 + q2, d0 = q0 - T(4) * q1 + T(3) * q2;
-  const T c1 = q1 - T(2) * q2 + q3, d1 = q1 - q3;
-  const T c2 = q2 - T(2) * q3 + q4, d2 = T(3) * q2 - T(4) * q3 + q4;
+  const T c0 = (s0 + s2) - T(2) * s1, d0 = (s0 - T(4) * s1) + T(3) * s2;
+  const T c1 = (s1 + s3) - T(2) * s2, d1 = s1 - s3;
+  const T c2 = (s2 + s4) - T(2) * s3, d2 = (T(3) * s2 - T(4) * s3) + s4;
   const T b0 = T(13.0 / 12.0) * (c0 * c0) + T(0.25) * (d0 * d0);
   const T b1 = T(13.0 / 12.0) * (c1 * c1) + T(0.25) * (d1 * d1);
   const T b2 = T(13.0 / 12.0) * (c2 * c2) + T(0.25) * (d2 * d2);
-  const T r0 = T(2.0 / 6.0) * q0 - T(7.0 / 6.0) * q1 + T(11.0 / 6.0) * q2;
-  const T r1 = -T(1.0 / 6.0) * q1 + T(5.0 / 6.0) * q2 + T(2.0 / 6.0) * q3;
-  const T r2 = T(2.0 / 6.0) * q2 + T(5.0 / 6.0) * q3 - T(1.0 / 6.0) * q4;
-  const double e0 = 1e-6 + double(b0), e1 = 1e-6 + double(b1), e2 = 1e-6 + double(b2);
+  const T r0 = T(2.0 / 6.0) * s0 - T(7.0 / 6.0) * s1 + T(11.0 / 6.0) * s2;
+  const T r1 = -T(1.0 / 6.0) * s1 + T(5.0 / 6.0) * s2 + T(2.0 / 6.0) * s3;
+  const T r2 = T(2.0 / 6.0) * s2 + T(5.0 / 6.0) * s3 - T(1.0 / 6.0) * s4;
+  const double v2 = double(v) * double(v);
+  const double e0 = 1e-6 + v2 * double(b0), e1 = 1e-6 + v2 * double(b1), e2 = 1e-6 + v2 * double(b2);
   const double e00 = e0 * e0, e11 = e1 * e1, e22 = e2 * e2;
   const double A0 = 0.1 * (e11 * e22), A1 = 0.6 * (e00 * e22), A2 = 0.3 * (e00 * e11);
   const double num = A0 * double(r0) + A1 * double(r1) + A2 * double(r2);
-  return T(num / (A0 + A1 + A2));
+  return T(double(v) * (num / (A0 + A1 + A2)));
 }
 
 }  // namespace uqlbm
