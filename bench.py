@@ -320,7 +320,9 @@ def main() -> int:
             s.close()
 
     if "riemann" in legs:
-        # --- 4096^2 Riemann, plain FP64 ------------------------------------
+        # --- 4096^2 Riemann, plain FP64 (the tile shape tuned on the batch) --
+        if tune:
+            lbm.set_flux_shape("fp64", tune["flux_tile_shape"])
         rc = cases.riemann(args.riemann_n, steps=args.riemann_steps)
         rs = lbm.Solver(rc, "fp64")
         rs.step(2)
