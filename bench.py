@@ -193,6 +193,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the served stream")
     ap.add_argument("--serve-tick", type=int, default=50, help="steps per server tick")
     ap.add_argument("--serve-max-wall", type=float, default=150.0)
+    ap.add_argument("--serve-sustained-rate", type=float, default=1.2,
+                    help="arrival rate of the second served stream (about half of one GPU's FP64 capacity)")
+    ap.add_argument("--serve-sustained-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--quick", action="store_true", help="small smoke-sized run")
     ap.add_argument("--no-extras", action="store_true", help="headline leg only")
@@ -223,6 +226,7 @@ def main() -> int:
         args.instances, args.steps, args.plain_steps = 16, 50, 5
         args.riemann_n, args.riemann_steps = 512, 5
         args.serve_seconds, args.cpu_seconds, args.accuracy_steps = 0.5, 3.0, 20
+        args.serve_sustained_seconds = 0
     if lbm.device_count() < 1:
         raise SystemExit("no gfx950 device visible")
     if args.flux_shape >= 0:
@@ -345,8 +349,14 @@ def main() -> int:
 
     if "serve" in legs and args.serve_seconds > 0:
         # --- served request stream ---------------------------------------
+        # the survey's stream (offered load ~21x one GPU's FP64 capacity),
+        # then the same request distribution at a rate one GPU sustains
         result["serve"] = serve_leg(args.serve_rate, args.serve_seconds, 7, args.serve_tick, args.serve_max_wall)
         log(f"serve: {result['serve']}")
+        if args.serve_sustained_seconds > 0:
+            result["serve_sustained"] = serve_leg(args.serve_sustained_rate, args.serve_sustained_seconds, 8,
+                                                  args.serve_tick, args.serve_max_wall)
+            log(f"serve (sustained rate): {result['serve_sustained']}")
 
     if "cpu" in legs:
         # --- CPU baseline: the oracle on this host ---------------------------
