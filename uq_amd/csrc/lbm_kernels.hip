@@ -559,13 +559,19 @@ __device__ void setup_region(Region<Sh>& R, const InstDev& in, int x0, int y0) {
   }
 }
 
+// An f value and its h partner, kept together in LDS so that one 8- or
+// 16-byte access moves both.
+template <typename TC>
+struct alignas(2 * sizeof(TC)) FH {
+  TC f, h;
+};
+
 template <typename Sh, typename TC, bool MOM = false>
 struct FluxStage {
-  TC f[Sh::RY][Sh::RX];
-  TC h[Sh::RY][Sh::RX];
-  TC fxf[Sh::TY][Sh::TX + 1], fxh[Sh::TY][Sh::TX + 1];   // x faces: face j is left of cell j
-  TC fyf[Sh::TY + 1][Sh::TX], fyh[Sh::TY + 1][Sh::TX];   // y faces: face j is below cell j
-  double mom[MOM ? 4 : 1][MOM ? Sh::Own : 1];            // per-cell sums of the stored dF, dH (MOM)
+  FH<TC> v[Sh::RY][Sh::RX];             // staged f, h of one direction
+  FH<TC> fx[Sh::TY][Sh::TX + 1];        // x faces: face j is left of cell j
+  FH<TC> fy[Sh::TY + 1][Sh::TX];        // y faces: face j is below cell j
+  double mom[MOM ? 4 : 1][MOM ? Sh::Own : 1];  // per-cell sums of the stored dF, dH (MOM)
 };
 
 // Reference stencil solid handling along one line (cpuLBM.cpp:1242-1322):
@@ -601,8 +607,9 @@ __device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const
   const unsigned mo = R.meta[ly + kH][lx + kH];
   if ((mo & kMetaType) == 2) {
     const TC a = TC(in.alpha[R.off[ly + kH][lx + kH]]);
-    vf = a * v * S.f[ly + kH][lx + kH];
-    vh = a * v * S.h[ly + kH][lx + kH];
+    const FH<TC> w = S.v[ly + kH][lx + kH];
+    vf = a * v * w.f;
+    vh = a * v * w.h;
     return;
   }
   const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
@@ -612,8 +619,9 @@ __device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       const int px = lx + (k - 2) * dx + kH, py = ly + (k - 2) * dy + kH;
-      Vf[k] = S.f[py][px];
-      Vh[k] = S.h[py][px];
+      const FH<TC> w = S.v[py][px];
+      Vf[k] = w.f;
+      Vh[k] = w.h;
       sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
       any |= sld[k];
     }
@@ -634,8 +642,9 @@ __device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int px = lx + (k - 1) * dx + kH, py = ly + (k - 1) * dy + kH;
-      Vf[k] = S.f[py][px];
-      Vh[k] = S.h[py][px];
+      const FH<TC> w = S.v[py][px];
+      Vf[k] = w.f;
+      Vh[k] = w.h;
       sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
       any |= sld[k];
     }
@@ -723,8 +732,7 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, const Regio
       const unsigned m = (unsigned)s_meta[j];
       const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
       const long long o = s_off[j];
-      (&S.f[0][0])[s_lds[j]] = TC(plane<TS>(src, P, d)[o]);
-      (&S.h[0][0])[s_lds[j]] = TC(plane<TS>(src, P, kQ + d)[o]);
+      (&S.v[0][0])[s_lds[j]] = FH<TC>{TC(plane<TS>(src, P, d)[o]), TC(plane<TS>(src, P, kQ + d)[o])};
     }
     __syncthreads();
     // faces: x face j of row r (owner column j - [sx > 0]); y face j of
@@ -734,16 +742,14 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, const Regio
       const TC v = TC(double(ex > 0 ? ex : -ex) * in.c);
       TC vf, vh;
       face_flux<0>(S, R, in, solids, j - (sx > 0 ? 1 : 0), row, sx, v, vf, vh);
-      S.fxf[row][j] = vf;
-      S.fxh[row][j] = vh;
+      S.fx[row][j] = FH<TC>{vf, vh};
     }
     if (ey && t < Sh::NYF) {
       const int col = t % Sh::TX, j = t / Sh::TX;
       const TC v = TC(double(ey > 0 ? ey : -ey) * in.c);
       TC vf, vh;
       face_flux<1>(S, R, in, solids, col, j - (sy > 0 ? 1 : 0), sy, v, vf, vh);
-      S.fyf[j][col] = vf;
-      S.fyh[j][col] = vh;
+      S.fy[j][col] = FH<TC>{vf, vh};
     }
     __syncthreads();
 #pragma unroll
@@ -756,28 +762,32 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, const Regio
       TC oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
       if (ex) {
         const int lo = sx > 0 ? -1 : 0;
-        oxf = S.fxf[ty][tx - lo];
-        oxh = S.fxh[ty][tx - lo];
+        const FH<TC> out = S.fx[ty][tx - lo];
+        oxf = out.f;
+        oxh = out.h;
         const int ux = tx - sx;
         if (R.meta[ty + kH][ux + kH] & kMetaClampX) {
           ixf = oxf;
           ixh = oxh;
         } else {
-          ixf = S.fxf[ty][ux - lo];
-          ixh = S.fxh[ty][ux - lo];
+          const FH<TC> in_ = S.fx[ty][ux - lo];
+          ixf = in_.f;
+          ixh = in_.h;
         }
       }
       if (ey) {
         const int lo = sy > 0 ? -1 : 0;
-        oyf = S.fyf[ty - lo][tx];
-        oyh = S.fyh[ty - lo][tx];
+        const FH<TC> out = S.fy[ty - lo][tx];
+        oyf = out.f;
+        oyh = out.h;
         const int uy = ty - sy;
         if (R.meta[uy + kH][tx + kH] & kMetaClampY) {
           iyf = oyf;
           iyh = oyh;
         } else {
-          iyf = S.fyf[uy - lo][tx];
-          iyh = S.fyh[uy - lo][tx];
+          const FH<TC> in_ = S.fy[uy - lo][tx];
+          iyf = in_.f;
+          iyh = in_.h;
         }
       }
       TC dF, dH;
