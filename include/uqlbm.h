@@ -198,7 +198,7 @@ typedef struct lbm_timing {
 /*
  * Flux tile shapes and the autotuner. The stencil kernels work on tiles of
  * own cells plus a 3-cell halo cross; shape 0 = 15 x 16 (256 threads),
- * 1 = 31 x 15 (512), 2 = 15 x 32 (512). lbm_batch_autotune times `steps`
+ * 1 = 31 x 15 (512), 2 = 15 x 32 (512), 3 = 39 x 12 (512), 4 = 23 x 20 (512). lbm_batch_autotune times `steps`
  * steps of the batch with each shape, keeps the fastest for the batch and
  * as the default of solvers, batches and servers created afterwards with
  * the batch's precision, restores the members' states (tuning changes no
@@ -207,7 +207,7 @@ typedef struct lbm_timing {
  * same for every shape. No counterpart in the reference (its stencil loops
  * are per cell).
  */
-#define LBM_FLUX_SHAPES 3
+#define LBM_FLUX_SHAPES 5
 int lbm_set_flux_shape(int32_t precision, int32_t shape);
 int lbm_flux_shape_dims(int32_t shape, int32_t* w, int32_t* h);
 int lbm_batch_autotune(lbm_batch* b, int32_t steps, int32_t* shape_out, double* ms_per_step_out);

@@ -133,3 +133,14 @@ def test_cpp_dropin_header_compiles(tmp_path):
     src.write_text('#include "gpuLBM.hpp"\nint main(){ gpuLBM l; l.Nx = 4; (void)l; return 0; }\n')
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                    check=True)
+
+
+def test_flux_shape_table():
+    """The flux tile shapes the autotuner chooses from (host query, no GPU)."""
+    dims = [lbm.flux_shape_dims(k) for k in range(lbm.FLUX_SHAPES)]
+    assert dims == [(15, 16), (31, 15), (15, 32), (39, 12), (23, 20)]
+    # every shape keeps one x face and one y face per thread of its block
+    for (w, h), nt in zip(dims, (256, 512, 512, 512, 512)):
+        assert (w + 1) * h <= nt and w * (h + 1) <= nt
+    with pytest.raises(lbm.LbmError):
+        lbm.flux_shape_dims(lbm.FLUX_SHAPES)

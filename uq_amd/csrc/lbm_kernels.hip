@@ -211,8 +211,10 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, TS* d
     s12<I>(r, hdt, TC(f[I]), TC(h[I]), fe, he, f2, h2);
     sumfe += double(fe);
     sumabs += fabs(double(fe));
-    dst[P * I + o] = TS(f2);
-    dst[P * (kQ + I) + o] = TS(h2);
+    // streaming stores: f2 and, below, f3 and G are written once per step
+    // and read back only by the next kernel, long after L2 has turned over
+    __builtin_nontemporal_store(TS(f2), &dst[P * I + o]);
+    __builtin_nontemporal_store(TS(h2), &dst[P * (kQ + I) + o]);
   };
   (one(std::integral_constant<int, Is>{}), ...);
   flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
@@ -304,10 +306,10 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     const TC h3 = ((Qh - hdt * A3) + ah3 * he3) * iah3;
     const TC gf = (Pf + TC(0.5) * dF) + af3 * (fe3 - f3);
     const TC gh = ((Qh + TC(0.5) * dH) + ah3 * (he3 - h3)) - hdt * A3;
-    f3b[P * I + o] = TS(f3);
-    f3b[P * (kQ + I) + o] = TS(h3);
-    out[P * I + o] = TS(gf);
-    out[P * (kQ + I) + o] = TS(gh);
+    __builtin_nontemporal_store(TS(f3), &f3b[P * I + o]);
+    __builtin_nontemporal_store(TS(h3), &f3b[P * (kQ + I) + o]);
+    __builtin_nontemporal_store(TS(gf), &out[P * I + o]);
+    __builtin_nontemporal_store(TS(gh), &out[P * (kQ + I) + o]);
   };
   (one(std::integral_constant<int, Is>{}), ...);
   flag_failure(in, step, m3.rho, sumfe, eq_check_tol<TC>(sumabs));
@@ -522,8 +524,11 @@ struct Shape {
 using Shape0 = Shape<15, 16, 256>;  // 240 own cells, 1.78 staged values per cell
 using Shape1 = Shape<31, 15, 512>;  // 465 own cells, 1.59
 using Shape2 = Shape<15, 32, 512>;  // 480 own cells, 1.59
+using Shape3 = Shape<39, 12, 512>;  // 468 own cells, 1.65 (312-byte FP64 row segments)
+using Shape4 = Shape<23, 20, 512>;  // 460 own cells, 1.56
 static_assert(Shape0::XF == 1 && Shape0::YF == 1 && Shape1::XF == 1 && Shape1::YF == 1 &&
-                  Shape2::XF == 1 && Shape2::YF == 1,
+                  Shape2::XF == 1 && Shape2::YF == 1 && Shape3::XF == 1 && Shape3::YF == 1 &&
+                  Shape4::XF == 1 && Shape4::YF == 1,
               "one x face and one y face per thread");
 
 template <typename Sh>
@@ -1110,6 +1115,8 @@ void flux_tile_dims(int shape, int* w, int* h) {
   switch (shape) {
     case 1: *w = Shape1::TX; *h = Shape1::TY; break;
     case 2: *w = Shape2::TX; *h = Shape2::TY; break;
+    case 3: *w = Shape3::TX; *h = Shape3::TY; break;
+    case 4: *w = Shape4::TX; *h = Shape4::TY; break;
     default: *w = Shape0::TX; *h = Shape0::TY; break;
   }
 }
@@ -1154,6 +1161,8 @@ static void prepare_shapes() {
   (void)StepGrids<TS, TC, Shape0>::get();
   (void)StepGrids<TS, TC, Shape1>::get();
   (void)StepGrids<TS, TC, Shape2>::get();
+  (void)StepGrids<TS, TC, Shape3>::get();
+  (void)StepGrids<TS, TC, Shape4>::get();
 }
 
 void prepare_launch(int fp64) {
@@ -1241,6 +1250,8 @@ void launch_step(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   switch (L.flux_shape) {
     case 1: launch_step_shape<TS, TC, Shape1>(L, s, par, tm); break;
     case 2: launch_step_shape<TS, TC, Shape2>(L, s, par, tm); break;
+    case 3: launch_step_shape<TS, TC, Shape3>(L, s, par, tm); break;
+    case 4: launch_step_shape<TS, TC, Shape4>(L, s, par, tm); break;
     default: launch_step_shape<TS, TC, Shape0>(L, s, par, tm); break;
   }
   k_tick<<<1, 64, 0, s>>>(L.step_ctr);

@@ -79,8 +79,8 @@ class KernelTimer {
 };
 
 // Flux tile shapes: 0 = 15 x 16 own cells (256 threads), 1 = 31 x 15 (512),
-// 2 = 15 x 32 (512).
-constexpr int kFluxShapes = 3;
+// 2 = 15 x 32 (512), 3 = 39 x 12 (512), 4 = 23 x 20 (512).
+constexpr int kFluxShapes = 5;
 void flux_tile_dims(int shape, int* w, int* h);
 int tile_halo();
 int dflux_planes();

@@ -94,12 +94,13 @@ def set_kernel_timing(enable: bool) -> None:
     lib().lbm_set_kernel_timing(1 if enable else 0)
 
 
-FLUX_SHAPES = 3
+FLUX_SHAPES = 5
 
 
 def set_flux_shape(precision: str, shape: int) -> None:
     """Flux tile shape of solvers, batches and servers of `precision`
-    created from now on (0: 15x16, 1: 31x15, 2: 15x32 own cells)."""
+    created from now on (0: 15x16, 1: 31x15, 2: 15x32, 3: 39x12, 4: 23x20
+    own cells)."""
     _check(lib().lbm_set_flux_shape(PRECISIONS[precision], int(shape)), "lbm_set_flux_shape")
 
 
