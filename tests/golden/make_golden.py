@@ -113,6 +113,47 @@ def make(name: str) -> str:
     return path
 
 
+# Jacobs' Mach-2 boundary-layer profiles at x = 0.9415 m, t = 7 ms, as the
+# reference digitised them from the published figure (pixel picks: origin,
+# a point on the value axis, a point on the height axis, then the curve;
+# Items/11.04.06-BoundaryLayer-Jacobs/invBLayer.m:49-55, scaled as its
+# manualCurve.m does: value axis 0..700 m/s or 200..270 K, height axis
+# 0..0.006 m, pixel rows counted downwards).
+JACOBS_U_PICKS = [160, 567, 160, 41, 1226, 572, 162, 566, 173, 555, 184, 544, 194, 532, 210, 514, 227, 497,
+                  248, 477, 268, 458, 285, 440, 304, 423, 327, 400, 356, 372, 379, 353, 402, 333, 424, 313,
+                  445, 296, 471, 276, 498, 256, 526, 237, 561, 215, 599, 194, 642, 173, 690, 156, 724, 147,
+                  764, 138, 809, 131, 855, 128, 899, 124, 953, 122, 985, 121, 1030, 122, 1076, 121, 1121, 122,
+                  1168, 121, 1181, 121]
+JACOBS_T_PICKS = [168, 584, 167, 30, 1234, 585, 169, 399, 185, 365, 202, 325, 220, 288, 237, 258, 260, 225,
+                  288, 189, 316, 163, 350, 142, 393, 129, 421, 128, 458, 136, 490, 151, 521, 168, 547, 184,
+                  577, 208, 607, 231, 631, 248, 657, 270, 686, 292, 725, 319, 761, 339, 793, 355, 829, 369,
+                  877, 383, 920, 393, 975, 401, 1023, 405, 1064, 406, 1112, 408, 1148, 409, 1190, 409]
+
+
+def scale_picks(picks, value0: float, value1: float, height1: float):
+    """Pixel picks -> (heights m, values): pairs (column, row) with rows
+    counted downwards; the first pair is the origin, the second lies on the
+    value axis at value1, the third on the height axis at height1."""
+    p = np.asarray(picks, dtype=np.float64)
+    col, row = p[0::2], -p[1::2]
+    col, row = col - col[0], row - row[0]
+    heights = col[3:] * (height1 / col[2])
+    values = value0 + row[3:] * ((value1 - value0) / row[1])
+    return heights, values
+
+
+def make_jacobs() -> str:
+    uh, uv = scale_picks(JACOBS_U_PICKS, 0.0, 700.0, 0.006)
+    th, tv = scale_picks(JACOBS_T_PICKS, 200.0, 270.0, 0.006)
+    path = os.path.join(HERE, "jacobs_profiles.json")
+    with open(path, "w") as fh:
+        json.dump({"x_m": 0.9415, "t_s": 7e-3,
+                   "ux": {"height_m": uh.round(7).tolist(), "value_m_s": uv.round(3).tolist()},
+                   "T": {"height_m": th.round(7).tolist(), "value_K": tv.round(3).tolist()}}, fh, indent=1)
+    print(f"jacobs profiles: {len(uh)} velocity and {len(th)} temperature points -> {os.path.relpath(path, ROOT)}")
+    return path
+
+
 def make_failure(name: str) -> str:
     from pyoracle import Oracle
     spec = spec_of(name)
@@ -131,5 +172,10 @@ def make_failure(name: str) -> str:
 
 
 if __name__ == "__main__":
-    for n in (sys.argv[1:] or list(FIXTURES) + list(FAILURES)):
-        make(n) if n in FIXTURES else make_failure(n)
+    for n in (sys.argv[1:] or list(FIXTURES) + list(FAILURES) + ["jacobs"]):
+        if n == "jacobs":
+            make_jacobs()
+        elif n in FIXTURES:
+            make(n)
+        else:
+            make_failure(n)

@@ -48,3 +48,20 @@ def test_failure_fixture_is_recorded():
     assert 0 < int(z["fail_step"]) < int(z["max_steps"])
     case = make_golden.build_case(str(z["spec"]))
     assert case.params["Tc_Tref"] == 1.0 and case.meta["mach"] > 0.4
+
+
+def test_jacobs_digitisation_meets_the_deck():
+    """The digitised Jacobs profiles, scaled as the reference scales them,
+    end at the reference deck's free stream: u -> 597.3 m/s and T -> 222 K
+    (a wrong axis scale or flipped row direction would miss both)."""
+    import json
+    from uq_amd import deck
+    d = json.load(open(os.path.join(GOLDEN, "jacobs_profiles.json")))
+    c = deck.read_mat_deck(os.path.join(GOLDEN, "jacobs_boundary_layer_input.mat"))
+    u_inf, t_inf = c.props["velocity_x"][0], c.props["temp"][0]
+    u = np.array(d["ux"]["value_m_s"])
+    t = np.array(d["T"]["value_K"])
+    assert abs(u[-5:].mean() - u_inf) < 0.01 * u_inf
+    assert abs(t[-4:].mean() - t_inf) < 0.5
+    assert u[0] < 5 and np.all(np.diff(u[:25]) > 0)          # no-slip wall, rising profile
+    assert t.max() > t_inf + 30                               # heating inside the layer
