@@ -98,3 +98,34 @@ def test_jacobs_boundary_layer(gpu_lib):
     assert rms_out <= 0.03 and rms_all <= 0.10
     assert 235.0 < T[1:4].max() < 270.0
     assert abs(T[int(round(5e-3 / info["dy"]))] - t_inf) < 3.0
+
+
+def test_couette_config_steady_profile(gpu_lib):
+    """The 64 x 64 Couette deck (u_w = 0.1, tau = 0.05, Pr = 0.71, WENO5,
+    IMEX RK3) run to steady state (t = 60) on the GPU: the velocity profile
+    is the slip Couette solution of the reference's own check
+    (Items/11.04.18 - Couette flow check for macrossan/couetteConstT.m)
+    within 3 % of u_wall, and its slope within 5 %. The scheme's steady state
+    carries a period-4 standing wiggle larger than the viscous-heating
+    temperature rise, so the temperature profile is not asserted (as in the
+    oracle's own check at 32 rows)."""
+    from uq_amd import cases
+    c = cases.couette(64)
+    s = gpu_lib.Solver(c, "fp64")
+    try:
+        n = int(60.0 / s.info()["dt"])
+        print(f"[golden] couette64 to steady state: {n} steps", flush=True)
+        assert s.step(n) == 0
+        m = s.get_macro()
+    finally:
+        s.close()
+    ys = np.arange(1, c.ny - 1)
+    u_sim = m["ux"][0, 1:c.ny - 1]
+    assert np.all(m["ux"] == m["ux"][:1, :])
+    u_ex, _, prm = cases.couette_analytic(c, ys)
+    dev = np.max(np.abs(u_sim - u_ex)) / c.meta["u_wall"]
+    slope = np.polyfit((ys - 0.5) / (c.ny - 2), u_sim, 1)[0]
+    print(f"[golden] couette64 steady: max |u - u_slip| / u_w {dev:.3f}, slope {slope:.4f} vs {prm['A']:.4f}",
+          flush=True)
+    assert dev <= 0.03
+    assert slope == pytest.approx(prm["A"], rel=0.05)
