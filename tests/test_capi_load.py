@@ -144,3 +144,27 @@ def test_flux_shape_table():
         assert (w + 1) * h <= nt and w * (h + 1) <= nt
     with pytest.raises(lbm.LbmError):
         lbm.flux_shape_dims(lbm.FLUX_SHAPES)
+
+
+def test_integration_binding_compiles(tmp_path):
+    """The maintainer-side binding shown in INTEGRATION.md (gpuStep.h)
+    compiles against include/uqlbm.h, given a class with the public members
+    of the reference's cpuLBM that it touches (declared here for the check
+    only)."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = re.search(r"```cpp\n(// gpuStep\.h.*?)```", text, re.S).group(1)
+    members = """
+#include <cstdio>
+struct double2 { double x, y; };
+struct cpuLBM {
+  int Nx, Ny; int* bndArray; int RKMETHOD, FMETHOD; double CFL, dtau, tt_tref, tt_time; int steps;
+  double tol; int periodicX, periodicY, mirrorNorth, mirrorSouth, mirrorEast, mirrorWest;
+  double gamma, R, mu, Pr, dx, dy, Lx, Ly, rho_ref, p_ref, T_ref, Tc_Tref;
+  int numProps; double *density, *pressure, *temp, *velocityX, *velocityY; int* cellType;
+  double *f_G, *h_G, *rho_G, *p_G, *T_G; double2* u_G; double residual; int step, nPrintOut, MAT;
+};
+"""
+    src = tmp_path / "binding.cpp"
+    src.write_text(members + block + "\nint main() { return 0; }\n")
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                   check=True)
