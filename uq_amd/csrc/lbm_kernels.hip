@@ -584,23 +584,28 @@ struct FluxStage {
 // the two nearest fluid points, and the point beyond it the same value. The
 // addressing equivalence with the reference is checked on the host per
 // geometry (lbm_runtime.hip, validate_geometry).
+// (The transition is located in the unrolled loop and the line updated in
+// the same unrolled iteration, so every index is a constant and the stencil
+// stays in registers.)
 template <int L, typename TC>
 __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) {
-  int diff = 0, jj = 0;
+  bool found = false;
 #pragma unroll
   for (int j = 0; j < L - 1; j++) {
     const int d = sld[j] - sld[j + 1];
-    if (d != 0 && diff == 0) {
-      diff = d;
-      jj = d > 0 ? j : j + 1;
+    if (!found && d != 0) {
+      found = true;
+      if (d > 0) {  // solid point j, fluid beyond it at j+1, j+2
+        const TC e = TC(2) * V[j + 1 < L ? j + 1 : L - 1] - V[j + 2 < L ? j + 2 : L - 1];
+        S[j] = e;
+        if (j - 1 >= 0) S[j - 1] = e;
+      } else {      // solid point j+1, fluid before it at j, j-1
+        const TC e = TC(2) * V[j] - V[j - 1 >= 0 ? j - 1 : 0];
+        S[j + 1] = e;
+        if (j + 2 < L) S[j + 2] = e;
+      }
     }
   }
-  int k1 = jj + diff, k2 = jj + 2 * diff;
-  k1 = k1 < 0 ? 0 : (k1 > L - 1 ? L - 1 : k1);
-  k2 = k2 < 0 ? 0 : (k2 > L - 1 ? L - 1 : k2);
-  S[jj] = TC(2) * V[k1] - V[k2];
-  const int nx = jj - diff;
-  if (nx >= 0 && nx < L) S[nx] = S[jj];
 }
 
 // Face flux (f and h) out of the owner at logical (lx, ly) along axis AX with
