@@ -571,13 +571,111 @@ struct alignas(2 * sizeof(TC)) FH {
   TC f, h;
 };
 
+// The staged f, h of one direction over the whole region, in region order
+// (row-major RY x RX, halo corners included) and padded to whole wave loads.
+// A wave load moves CH positions with LPP lanes per position, one dword per
+// lane, straight from global memory into LDS.
+template <typename Sh, typename TS>
+struct StageBuf {
+  static constexpr int LPP = int(sizeof(FH<TS>) / 4);
+  static constexpr int CH = 64 / LPP;
+  static constexpr int NCH = (Sh::RY * Sh::RX + CH - 1) / CH;
+  FH<TS> v[NCH * CH];
+};
+
 template <typename Sh, typename TC, bool MOM = false>
 struct FluxStage {
-  FH<TC> v[Sh::RY][Sh::RX];             // staged f, h of one direction
-  FH<TC> fx[Sh::TY][Sh::TX + 1];        // x faces: face j is left of cell j
-  FH<TC> fy[Sh::TY + 1][Sh::TX];        // y faces: face j is below cell j
+  FH<TC> fx[Sh::TY][Sh::TX + 1];  // x faces: face j is left of cell j
+  FH<TC> fy[Sh::TY + 1][Sh::TX];  // y faces: face j is below cell j
   double mom[MOM ? 4 : 1][MOM ? Sh::Own : 1];  // per-cell sums of the stored dF, dH (MOM)
 };
+
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int kWaitVmem = 0x0F70;  // s_waitcnt vmcnt(0): this wave's loads (also those into LDS) done
+
+// Issue the loads of direction I's f, h over the region into B, without
+// waiting for them: each wave takes every (NT/64)-th group of CH positions;
+// the first LPP/2 lanes of a position load the f value of the
+// mirror-mapped direction, the others its h value. Directions without a y
+// component skip the groups that lie wholly in the top or bottom halo rows.
+// A wave's loads have landed once it passes s_waitcnt vmcnt(0).
+template <typename TS, typename Sh>
+__device__ __forceinline__ void stage_dir(StageBuf<Sh, TS>& B, const Region<Sh>& R, const void* src,
+                                          long long P, int I) {
+  using SB = StageBuf<Sh, TS>;
+  constexpr int NPos = Sh::RY * Sh::RX, NW = Sh::NT / 64;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int sub = lane / SB::LPP, q = lane % SB::LPP;
+  const int comp = q / (SB::LPP / 2), word = q % (SB::LPP / 2);
+  const bool ybands = c_lat_y[I] != 0;
+  const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
+  const char* base = reinterpret_cast<const char*>(src);
+  for (int c = w; c < SB::NCH; c += NW) {
+    if (!ybands && (c * SB::CH + SB::CH <= kH * Sh::RX || c * SB::CH >= (Sh::RY - kH) * Sh::RX)) continue;
+    const int p = c * SB::CH + sub, pc = p < NPos ? p : 0;  // padding lanes reload position 0
+    const unsigned m = (&R.meta[0][0])[pc];
+    const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
+    const long long e = (long long)(d + comp * kQ) * P + (&R.off[0][0])[pc];
+    const char* g = base + e * (long long)sizeof(TS) + word * 4;
+    __builtin_amdgcn_global_load_lds((global_void*)g, (lds_void*)&B.v[c * SB::CH], 4, 0, 0);
+  }
+}
+
+// Synchronous staging through registers, for the kernels whose LDS cannot
+// hold a second buffer: slot r = t + j NT of thread t is an own cell (r <
+// Own), an x halo band position (the next NXB) or a y halo band position
+// (the next NYB); a direction stages the own slots and the bands of its
+// nonzero axes. Each slot keeps its LDS position, grid offset and meta byte
+// for the whole tile.
+template <typename Sh>
+struct SlotSet {
+  int lds[Sh::Slots], off[Sh::Slots], meta[Sh::Slots];
+};
+
+template <typename Sh>
+__device__ __forceinline__ void setup_slots(SlotSet<Sh>& L, const Region<Sh>& R) {
+#pragma unroll
+  for (int j = 0; j < Sh::Slots; j++) {
+    const int r = threadIdx.x + j * Sh::NT;
+    int lx = 0, ly = 0, kind = 3;  // 0 own, 1 x band, 2 y band, 3 none
+    if (r < Sh::Own) {
+      lx = r % Sh::TX;
+      ly = r / Sh::TX;
+      kind = 0;
+    } else if (r < Sh::Own + Sh::NXB) {
+      const int k = r - Sh::Own, side = k / (kH * Sh::TY), kk = k % (kH * Sh::TY);
+      ly = kk / kH;
+      lx = side ? Sh::TX + kk % kH : -kH + kk % kH;
+      kind = 1;
+    } else if (r < Sh::Own + Sh::NXB + Sh::NYB) {
+      const int k = r - Sh::Own - Sh::NXB, side = k / (kH * Sh::TX), kk = k % (kH * Sh::TX);
+      lx = kk % Sh::TX;
+      ly = side ? Sh::TY + kk / Sh::TX : -kH + kk / Sh::TX;
+      kind = 2;
+    }
+    L.lds[j] = (ly + kH) * Sh::RX + (lx + kH);
+    L.off[j] = kind < 3 ? R.off[ly + kH][lx + kH] : 0;
+    L.meta[j] = kind < 3 ? (int)R.meta[ly + kH][lx + kH] | (kind << 8) : (3 << 8);
+  }
+}
+
+template <typename TS, typename Sh>
+__device__ __forceinline__ void stage_slots(StageBuf<Sh, TS>& B, const SlotSet<Sh>& L, const void* src,
+                                            long long P, int I) {
+  const int ex = c_lat_x[I], ey = c_lat_y[I];
+  const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
+#pragma unroll
+  for (int j = 0; j < Sh::Slots; j++) {
+    const int kind = L.meta[j] >> 8;
+    if (kind == 3 || (kind == 1 && !ex) || (kind == 2 && !ey)) continue;
+    const unsigned m = (unsigned)L.meta[j];
+    const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
+    const long long o = L.off[j];
+    B.v[L.lds[j]] = FH<TS>{plane<TS>(const_cast<void*>(src), P, d)[o], plane<TS>(const_cast<void*>(src), P, kQ + d)[o]};
+  }
+}
 
 // Reference stencil solid handling along one line (cpuLBM.cpp:1242-1322):
 // the first solid/fluid transition's solid point becomes 2 V(m1) - V(m2) of
@@ -609,27 +707,27 @@ __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) 
 }
 
 // Face flux (f and h) out of the owner at logical (lx, ly) along axis AX with
-// stencil step sg and speed v = |e_axis|: the wall closure at solid owners,
-// else WENO5/NND on the staged line.
-template <int AX, typename Sh, typename TC, bool MOM>
-__device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
+// stencil step sg and speed v = |e_axis|, from the staged direction V: the
+// wall closure at solid owners, else WENO5/NND on the staged line.
+template <int AX, typename Sh, typename TC>
+__device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, const double* alpha, int method,
                                           bool solids, int lx, int ly, int sg, TC v, TC& vf, TC& vh) {
   const unsigned mo = R.meta[ly + kH][lx + kH];
   if ((mo & kMetaType) == 2) {
-    const TC a = TC(in.alpha[R.off[ly + kH][lx + kH]]);
-    const FH<TC> w = S.v[ly + kH][lx + kH];
+    const TC a = TC(alpha[R.off[ly + kH][lx + kH]]);
+    const FH<TC> w = V[(ly + kH) * Sh::RX + lx + kH];
     vf = a * v * w.f;
     vh = a * v * w.h;
     return;
   }
   const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
-  if (in.flux_method == 1) {
+  if (method == 1) {
     TC Vf[5], Vh[5];
     int sld[5], any = 0;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       const int px = lx + (k - 2) * dx + kH, py = ly + (k - 2) * dy + kH;
-      const FH<TC> w = S.v[py][px];
+      const FH<TC> w = V[py * Sh::RX + px];
       Vf[k] = w.f;
       Vh[k] = w.h;
       sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
@@ -652,7 +750,7 @@ __device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int px = lx + (k - 1) * dx + kH, py = ly + (k - 1) * dy + kH;
-      const FH<TC> w = S.v[py][px];
+      const FH<TC> w = V[py * Sh::RX + px];
       Vf[k] = w.f;
       Vh[k] = w.h;
       sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
@@ -679,174 +777,205 @@ __device__ __forceinline__ void face_flux(const FluxStage<Sh, TC, MOM>& S, const
 //   MODE 2: source the current state (EULER, EULER_EQ); dF, dH -> dF planes.
 // MOM (with MODE 0 of RK3): also the moments of each fluid cell's stored
 // dF, dH over the directions -> in.dmom.
+//
+// Directions alternate between two staging buffers: while the threads
+// compute direction I's faces from one, the loads of direction I+1 stream
+// into the other, so the load latency hides behind the face arithmetic.
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
-__device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
-                                          const TileRef& tr, int par) {
-  const int t = threadIdx.x;
-  const bool solids = tr.flags & kTileSolid;
-  const long long P = in.plane;
-  void* src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
-  TS* dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
-  const TC rdx = TC(in.rdx), rdy = TC(in.rdy);
-  // own cells of this thread: cell ci = t + q NT at (ci % TX, ci / TX)
+
+// Per-thread state of one tile pass. The instance fields the direction loop
+// needs are copied here once per tile: read through the instance table
+// inside the loop (after the loop's stores) they would be vector loads,
+// whose wait would also wait for the staging loads in flight.
+template <typename TS, typename TC, typename Sh>
+struct TileWork {
   bool fluid[Sh::Cells];
   int oo[Sh::Cells];
+  bool solids;
+  int method;
+  const void* src;
+  TS* dst;
+  const double* alpha;
+  long long P;
+  double c;
+  TC rdx, rdy;
+};
+
+// Direction I of a tile.
+//   DB (two staging buffers): faces from cur, landed by the barrier on
+//   entry, while the loads of direction I+1 (`next`) stream into nxt.
+//   One buffer (cur and nxt the same): the threads stage direction I
+//   through registers on entry.
+template <typename TS, typename TC, int MODE, bool MOM, bool DB, typename Sh>
+__device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
+                                         const TileWork<TS, TC, Sh>& W, const SlotSet<Sh>& L,
+                                         StageBuf<Sh, TS>& cur, StageBuf<Sh, TS>& nxt, int I, bool next) {
+  const int t = threadIdx.x;
+  const long long P = W.P;
+  // with no loads in flight across the faces the instance fields are read
+  // where used, which keeps them out of the registers the faces need
+  const double c = DB ? W.c : in.c;
+  const int method = DB ? W.method : in.flux_method;
+  const double* alpha = DB ? W.alpha : in.alpha;
+  const int ex = c_lat_x[I], ey = c_lat_y[I];
+  const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
+  if constexpr (!DB) stage_slots<TS>(cur, L, W.src, P, I);  // cur was last read before the previous barrier
+  __syncthreads();  // cur complete; fx, fy (DB: nxt) free
+  if (DB && next) stage_dir<TS>(nxt, R, W.src, P, I + 1);
+  // faces: x face j of row r (owner column j - [sx > 0]); y face j of
+  // column c (owner row j - [sy > 0])
+  if (ex && t < Sh::NXF) {
+    const int row = t / (Sh::TX + 1), j = t % (Sh::TX + 1);
+    const TC v = TC(double(ex > 0 ? ex : -ex) * c);
+    TC vf, vh;
+    face_flux<0>(cur.v, R, alpha, method, W.solids, j - (sx > 0 ? 1 : 0), row, sx, v, vf, vh);
+    S.fx[row][j] = FH<TC>{vf, vh};
+  }
+  if (ey && t < Sh::NYF) {
+    const int col = t % Sh::TX, j = t / Sh::TX;
+    const TC v = TC(double(ey > 0 ? ey : -ey) * c);
+    TC vf, vh;
+    face_flux<1>(cur.v, R, alpha, method, W.solids, col, j - (sy > 0 ? 1 : 0), sy, v, vf, vh);
+    S.fy[j][col] = FH<TC>{vf, vh};
+  }
+  if constexpr (DB) __builtin_amdgcn_s_waitcnt(kWaitVmem);  // this wave's loads into nxt landed
+  __syncthreads();
+  const TC rdx = W.rdx, rdy = W.rdy;
+#pragma unroll
+  for (int q = 0; q < Sh::Cells; q++) {
+    if (!W.fluid[q]) continue;
+    const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
+    // reference cFLUX (cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) +
+    // (dt/dy)(Fy_out - Fy_in), grouped out - in; an upwind owner across a
+    // clamped edge is the cell itself
+    TC oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
+    if (ex) {
+      const int lo = sx > 0 ? -1 : 0;
+      const FH<TC> out = S.fx[ty][tx - lo];
+      oxf = out.f;
+      oxh = out.h;
+      const int ux = tx - sx;
+      if (R.meta[ty + kH][ux + kH] & kMetaClampX) {
+        ixf = oxf;
+        ixh = oxh;
+      } else {
+        const FH<TC> in_ = S.fx[ty][ux - lo];
+        ixf = in_.f;
+        ixh = in_.h;
+      }
+    }
+    if (ey) {
+      const int lo = sy > 0 ? -1 : 0;
+      const FH<TC> out = S.fy[ty - lo][tx];
+      oyf = out.f;
+      oyh = out.h;
+      const int uy = ty - sy;
+      if (R.meta[uy + kH][tx + kH] & kMetaClampY) {
+        iyf = oyf;
+        iyh = oyh;
+      } else {
+        const FH<TC> in_ = S.fy[uy - lo][tx];
+        iyf = in_.f;
+        iyh = in_.h;
+      }
+    }
+    TC dF, dH;
+    if (ex && ey) {
+      dF = (rdx * oxf + rdy * oyf) - (rdx * ixf + rdy * iyf);
+      dH = (rdx * oxh + rdy * oyh) - (rdx * ixh + rdy * iyh);
+    } else if (ex) {
+      dF = rdx * oxf - rdx * ixf;
+      dH = rdx * oxh - rdx * ixh;
+    } else {
+      dF = rdy * oyf - rdy * iyf;
+      dH = rdy * oyh - rdy * iyh;
+    }
+    const long long o = W.oo[q];
+    if (MODE != 1) {
+      W.dst[P * (I - 1) + o] = TS(dF);
+      W.dst[P * (12 + I - 1) + o] = TS(dH);
+      if (MOM) {
+        const double dFs = double(TS(dF));
+        S.mom[0][ci] += dFs;
+        S.mom[1][ci] += dFs * (ex * c);
+        S.mom[2][ci] += dFs * (ey * c);
+        S.mom[3][ci] += double(TS(dH));
+      }
+    } else {
+      TS* pf = W.dst + P * I + o;
+      TS* ph = W.dst + P * (kQ + I) + o;
+      *pf = TS(TC(*pf) - TC(0.5) * dF);
+      *ph = TS(TC(*ph) - TC(0.5) * dH);
+    }
+  }
+}
+
+// FP64 with the moment sums: one staging buffer, so that three 8-wave blocks
+// fit a CU's LDS; otherwise two.
+template <typename TS, bool MOM>
+struct TwoBuffers {
+  static constexpr bool value = !(MOM && sizeof(TS) == 8);
+};
+
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+__device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh, TS>& VA, StageBuf<Sh, TS>& VB,
+                                          const Region<Sh>& R, const InstDev& in, const TileRef& tr, int par) {
+  static_assert(std::is_same<TS, TC>::value, "LDS holds the stored values as computed on");
+  static_assert((kQ - 1) % 2 == 0, "directions pair up over the two staging buffers");
+  constexpr bool DB = TwoBuffers<TS, MOM>::value;
+  const int t = threadIdx.x;
+  TileWork<TS, TC, Sh> W;
+  W.solids = tr.flags & kTileSolid;
+  W.method = in.flux_method;
+  W.alpha = in.alpha;
+  W.c = in.c;
+  W.P = in.plane;
+  W.src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
+  W.dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
+  W.rdx = TC(in.rdx);
+  W.rdy = TC(in.rdy);
+  // own cells of this thread: cell ci = t + q NT at (ci % TX, ci / TX)
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
     const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
     const bool own = ci < Sh::Own && tr.tx * Sh::TX + tx < in.nx && tr.ty * Sh::TY + ty < in.ny;
-    fluid[q] = own && (R.meta[ty + kH][tx + kH] & kMetaType) == 0;
-    oo[q] = own ? R.off[ty + kH][tx + kH] : 0;
+    W.fluid[q] = own && (R.meta[ty + kH][tx + kH] & kMetaType) == 0;
+    W.oo[q] = own ? R.off[ty + kH][tx + kH] : 0;
     if (MOM && ci < Sh::Own) {
 #pragma unroll
       for (int k = 0; k < 4; k++) S.mom[k][ci] = 0.0;
     }
   }
-  // Staging slots of this thread, fixed for the tile: slot r < Own is an own
-  // cell, then the x halo bands (NXB), then the y halo bands (NYB). Each slot
-  // keeps its LDS position, grid offset and meta byte; a direction stages the
-  // own slots and the bands along its nonzero axes.
-  int s_lds[Sh::Slots], s_off[Sh::Slots], s_meta[Sh::Slots];
-#pragma unroll
-  for (int j = 0; j < Sh::Slots; j++) {
-    const int r = t + j * Sh::NT;
-    int lx = 0, ly = 0, kind = 3;  // 0 own, 1 x band, 2 y band, 3 none
-    if (r < Sh::Own) {
-      lx = r % Sh::TX;
-      ly = r / Sh::TX;
-      kind = 0;
-    } else if (r < Sh::Own + Sh::NXB) {
-      const int k = r - Sh::Own, side = k / (kH * Sh::TY), kk = k % (kH * Sh::TY);
-      ly = kk / kH;
-      lx = side ? Sh::TX + kk % kH : -kH + kk % kH;
-      kind = 1;
-    } else if (r < Sh::Own + Sh::NXB + Sh::NYB) {
-      const int k = r - Sh::Own - Sh::NXB, side = k / (kH * Sh::TX), kk = k % (kH * Sh::TX);
-      lx = kk % Sh::TX;
-      ly = side ? Sh::TY + kk / Sh::TX : -kH + kk / Sh::TX;
-      kind = 2;
+  SlotSet<Sh> L;
+  if constexpr (DB) {
+    stage_dir<TS>(VA, R, W.src, W.P, 1);
+    __builtin_amdgcn_s_waitcnt(kWaitVmem);
+    for (int I = 1; I < kQ; I += 2) {
+      flux_dir<TS, TC, MODE, MOM, true>(S, R, in, W, L, VA, VB, I, true);
+      flux_dir<TS, TC, MODE, MOM, true>(S, R, in, W, L, VB, VA, I + 1, I + 2 < kQ);
     }
-    s_lds[j] = (ly + kH) * Sh::RX + (lx + kH);
-    s_off[j] = kind < 3 ? R.off[ly + kH][lx + kH] : 0;
-    s_meta[j] = kind < 3 ? (int)R.meta[ly + kH][lx + kH] | (kind << 8) : (3 << 8);
-  }
-  for (int I = 1; I < kQ; I++) {
-    const int ex = c_lat_x[I], ey = c_lat_y[I];
-    const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
-    const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
-#pragma unroll
-    for (int j = 0; j < Sh::Slots; j++) {
-      const int kind = s_meta[j] >> 8;
-      if (kind == 3 || (kind == 1 && !ex) || (kind == 2 && !ey)) continue;
-      const unsigned m = (unsigned)s_meta[j];
-      const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
-      const long long o = s_off[j];
-      (&S.v[0][0])[s_lds[j]] = FH<TC>{TC(plane<TS>(src, P, d)[o]), TC(plane<TS>(src, P, kQ + d)[o])};
-    }
-    __syncthreads();
-    // faces: x face j of row r (owner column j - [sx > 0]); y face j of
-    // column c (owner row j - [sy > 0])
-    if (ex && t < Sh::NXF) {
-      const int row = t / (Sh::TX + 1), j = t % (Sh::TX + 1);
-      const TC v = TC(double(ex > 0 ? ex : -ex) * in.c);
-      TC vf, vh;
-      face_flux<0>(S, R, in, solids, j - (sx > 0 ? 1 : 0), row, sx, v, vf, vh);
-      S.fx[row][j] = FH<TC>{vf, vh};
-    }
-    if (ey && t < Sh::NYF) {
-      const int col = t % Sh::TX, j = t / Sh::TX;
-      const TC v = TC(double(ey > 0 ? ey : -ey) * in.c);
-      TC vf, vh;
-      face_flux<1>(S, R, in, solids, col, j - (sy > 0 ? 1 : 0), sy, v, vf, vh);
-      S.fy[j][col] = FH<TC>{vf, vh};
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < Sh::Cells; q++) {
-      if (!fluid[q]) continue;
-      const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
-      // reference cFLUX (cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) +
-      // (dt/dy)(Fy_out - Fy_in), grouped out - in; an upwind owner across a
-      // clamped edge is the cell itself
-      TC oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
-      if (ex) {
-        const int lo = sx > 0 ? -1 : 0;
-        const FH<TC> out = S.fx[ty][tx - lo];
-        oxf = out.f;
-        oxh = out.h;
-        const int ux = tx - sx;
-        if (R.meta[ty + kH][ux + kH] & kMetaClampX) {
-          ixf = oxf;
-          ixh = oxh;
-        } else {
-          const FH<TC> in_ = S.fx[ty][ux - lo];
-          ixf = in_.f;
-          ixh = in_.h;
-        }
-      }
-      if (ey) {
-        const int lo = sy > 0 ? -1 : 0;
-        const FH<TC> out = S.fy[ty - lo][tx];
-        oyf = out.f;
-        oyh = out.h;
-        const int uy = ty - sy;
-        if (R.meta[uy + kH][tx + kH] & kMetaClampY) {
-          iyf = oyf;
-          iyh = oyh;
-        } else {
-          const FH<TC> in_ = S.fy[uy - lo][tx];
-          iyf = in_.f;
-          iyh = in_.h;
-        }
-      }
-      TC dF, dH;
-      if (ex && ey) {
-        dF = (rdx * oxf + rdy * oyf) - (rdx * ixf + rdy * iyf);
-        dH = (rdx * oxh + rdy * oyh) - (rdx * ixh + rdy * iyh);
-      } else if (ex) {
-        dF = rdx * oxf - rdx * ixf;
-        dH = rdx * oxh - rdx * ixh;
-      } else {
-        dF = rdy * oyf - rdy * iyf;
-        dH = rdy * oyh - rdy * iyh;
-      }
-      const long long o = oo[q];
-      if (MODE != 1) {
-        dst[P * (I - 1) + o] = TS(dF);
-        dst[P * (12 + I - 1) + o] = TS(dH);
-        if (MOM) {
-          const double dFs = double(TS(dF));
-          S.mom[0][ci] += dFs;
-          S.mom[1][ci] += dFs * (ex * in.c);
-          S.mom[2][ci] += dFs * (ey * in.c);
-          S.mom[3][ci] += double(TS(dH));
-        }
-      } else {
-        TS* pf = dst + P * I + o;
-        TS* ph = dst + P * (kQ + I) + o;
-        *pf = TS(TC(*pf) - TC(0.5) * dF);
-        *ph = TS(TC(*ph) - TC(0.5) * dH);
-      }
-    }
+  } else {
+    setup_slots(L, R);
+    for (int I = 1; I < kQ; I++) flux_dir<TS, TC, MODE, MOM, false>(S, R, in, W, L, VA, VA, I, false);
   }
   if (MOM) {
 #pragma unroll
     for (int q = 0; q < Sh::Cells; q++) {
-      if (!fluid[q]) continue;
+      if (!W.fluid[q]) continue;
       const int ci = t + q * Sh::NT;
-      const long long o = oo[q];
+      const long long o = W.oo[q];
       in.dmom[o] = S.mom[0][ci];
-      in.dmom[P + o] = S.mom[1][ci];
-      in.dmom[2 * P + o] = S.mom[2][ci];
-      in.dmom[3 * P + o] = S.mom[3][ci];
+      in.dmom[W.P + o] = S.mom[1][ci];
+      in.dmom[2 * W.P + o] = S.mom[2][ci];
+      in.dmom[3 * W.P + o] = S.mom[3][ci];
     }
   }
 }
 
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                           int step, Region<Sh>& R, FluxStage<Sh, TC, MOM>& S) {
+                                           int step, Region<Sh>& R, FluxStage<Sh, TC, MOM>& S,
+                                           StageBuf<Sh, TS>& VA, StageBuf<Sh, TS>& VB) {
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
@@ -854,20 +983,23 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
     __syncthreads();
     setup_region(R, in, tr.tx * Sh::TX, tr.ty * Sh::TY);
     __syncthreads();
-    flux_tile<TS, TC, MODE, MOM>(S, R, in, tr, par);
+    flux_tile<TS, TC, MODE, MOM>(S, VA, VB, R, in, tr, par);
   }
 }
 
 // FP32 storage: the compiler's own register allocation. FP64: capped at 80
-// registers for 6 waves per SIMD, which the moment-summing stage-2 variant
-// would otherwise miss (83 registers, 5 waves; 4096^2: 4.18 -> 3.88 ms per
-// launch).
+// registers for 6 waves per SIMD.
+// (The staging buffers are separate LDS variables so that the compiler sees
+// that reads of one do not wait for the loads into the other; the second
+// is dropped where it is unused.)
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 __global__ __launch_bounds__(Sh::NT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
                                                  const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, TC, MOM> S;
-  flux_tiles<TS, TC, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S);
+  __shared__ StageBuf<Sh, TS> VA;
+  __shared__ StageBuf<Sh, TS> VB;
+  flux_tiles<TS, TC, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S, VA, VB);
 }
 
 template <int MODE, bool MOM, typename Sh>
@@ -875,7 +1007,9 @@ __global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(6))) voi
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, double, MOM> S;
-  flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S);
+  __shared__ StageBuf<Sh, double> VA;
+  __shared__ StageBuf<Sh, double> VB;
+  flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S, VA, VB);
 }
 
 // The flux kernel of a storage/compute type pair and tile shape.
