@@ -9,9 +9,10 @@
 //              -> f2 (cpuLBM.cpp:1819-1924)
 //  [k_alpha    wall cells: closure factor from the fluid neighbours' face
 //              fluxes of f2 (wallFluxes, cpuLBM.cpp:1502-1590)]
-//   k_flux<0>  stencil: per direction, f2 of tile + 3-cell halo cross in LDS,
+//   k_flux<0>  stencil: per direction, f2 of tile + 3-cell halo in LDS,
 //              x/y face fluxes (WENO5 or NND), net flux dF(f2) -> dF planes
-//              (posFluxes + cFLUX, cpuLBM.cpp:1436-1500)
+//              and the moment sums of dF (posFluxes + cFLUX,
+//              cpuLBM.cpp:1436-1500)
 //   k_s3       pointwise: moments of f - dF(f2), stage-3 equilibrium,
 //              implicit stage 3 -> f3, and the final combination except its
 //              last flux term -> the other state buffer (cpuLBM.cpp:1926-2042)
@@ -19,10 +20,12 @@
 //   k_flux<1>  stencil: net flux dF(f3); new state -= dF(f3)/2
 //
 // The stencil kernels run one direction at a time through a runtime loop
-// (one copy of the flux code, low register use) on 15 x 16 tiles with 256
-// threads: every thread computes exactly one x face and one y face per
-// direction (16 x 16 x-face owners, 15 x 17 y-face owners), so no wave
-// waits on a straggler at the barriers.
+// (one copy of the flux code, low register use) on tiles of one of five
+// shapes (the autotuner picks one per launch group): every thread computes
+// at most one x face and one y face per direction, so no wave waits on a
+// straggler at the barriers. FP32 storage stages the next direction into a
+// second LDS buffer with direct-to-LDS loads while the faces of the current
+// one are computed; FP64 stages through registers.
 #include <climits>
 #include <type_traits>
 
@@ -910,11 +913,15 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   }
 }
 
-// FP64 with the moment sums: one staging buffer, so that three 8-wave blocks
-// fit a CU's LDS; otherwise two.
+// Staging per storage type, as measured on the two workloads (4096^2
+// Riemann, 256-member Couette ensemble): FP32 two buffers with loads straight
+// into LDS (stage-3 flux 3.85 -> 3.56 ms on the ensemble); FP64 one buffer
+// through registers, which is as fast or faster there (4.75-4.77 against
+// 4.87 ms) and leaves room for three 8-wave blocks per CU with the moment
+// sums.
 template <typename TS, bool MOM>
 struct TwoBuffers {
-  static constexpr bool value = !(MOM && sizeof(TS) == 8);
+  static constexpr bool value = sizeof(TS) != 8;
 };
 
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
