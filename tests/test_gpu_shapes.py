@@ -49,6 +49,27 @@ def test_shapes_bitwise_equal(gpu_lib, case):
         assert np.array_equal(f, ref[1]) and np.array_equal(h, ref[2]), shape
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_shapes_keep_uniform_deck_uniform(gpu_lib, precision):
+    """A Couette deck is uniform along x; whichever face path (tiles with or
+    without solid cells) each tile takes, every column must round the same,
+    or an x-dependent mode is seeded (on this deck it grows ~10^4 per 100
+    steps)."""
+    c = cases.couette(64, steps=40)
+    for shape in range(gpu_lib.FLUX_SHAPES):
+        gpu_lib.set_flux_shape(precision, shape)
+        try:
+            s = gpu_lib.Solver(c, precision)
+            assert s.step(40) == 0
+            f, h = s.get_dist()
+            s.close()
+        finally:
+            gpu_lib.set_flux_shape(precision, 0)
+        print(f"[shapes] uniform deck {precision} shape {shape}", flush=True)
+        assert np.array_equal(f, np.broadcast_to(f[:1], f.shape)), shape
+        assert np.array_equal(h, np.broadcast_to(h[:1], h.shape)), shape
+
+
 def test_shapes_match_oracle(gpu_lib):
     from pyoracle import Oracle
     c = cases.couette(33, nx=70, u_wall=0.12, steps=10)

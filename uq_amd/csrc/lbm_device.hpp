@@ -200,10 +200,18 @@ __device__ __forceinline__ T minmod(T a, T b) {
   return a + b;  // NaN input only
 }
 
+__device__ __forceinline__ float fused(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fused(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// The reconstructions are written with every fused multiply-add explicit and
+// contraction off, so each inlined copy (the solid and solid-free face paths,
+// every tile shape) rounds identically: a deck that is uniform along an axis
+// stays bitwise uniform whichever path each tile takes.
 template <typename T>
 __device__ __forceinline__ T nnd(T v, T s0, T s1, T s2) {
+#pragma clang fp contract(off)
   const T fi = v * s1, fp = v * s2, fm = v * s0;
-  return fi + T(0.5) * minmod(fp - fi, fi - fm);
+  return fused(T(0.5), minmod(fp - fi, fi - fm), fi);
 }
 
 // Jiang-Shu WENO5 with d = (0.1, 0.6, 0.3), eps = 1e-6 on |e|*S. The
@@ -212,23 +220,37 @@ __device__ __forceinline__ T nnd(T v, T s0, T s1, T s2) {
 // normalised weights w_k = a_k / sum(a) with a_k = d_k/(eps+b_k)^2 are
 // formed as A_k / sum(A) with A_k = d_k * prod_{j!=k} (eps+b_j)^2 — the
 // same ratio, one division instead of six — in FP64 for both precisions.
+// The quotient num / sum(A) is formed by Newton-Raphson refinement of the
+// hardware reciprocal (two steps r += r (1 - D r), then the residual
+// correction q += r (num - D q), Markstein's scheme): every A_k >= 0.1e-24
+// (each eps + b_j >= 1e-6), so sum(A) is a normal number far from both
+// range limits, the range scaling of a general division is not needed, and
+// the result is within one rounding of the quotient.
 template <typename T>
 __device__ __forceinline__ T weno5(T v, T s0, T s1, T s2, T s3, T s4) {
-  const T c0 = (s0 + s2) - T(2) * s1, d0 = (s0 - T(4) * s1) + T(3) * s2;
-  const T c1 = (s1 + s3) - T(2) * s2, d1 = s1 - s3;
-  const T c2 = (s2 + s4) - T(2) * s3, d2 = (T(3) * s2 - T(4) * s3) + s4;
-  const T b0 = T(13.0 / 12.0) * (c0 * c0) + T(0.25) * (d0 * d0);
-  const T b1 = T(13.0 / 12.0) * (c1 * c1) + T(0.25) * (d1 * d1);
-  const T b2 = T(13.0 / 12.0) * (c2 * c2) + T(0.25) * (d2 * d2);
-  const T r0 = T(2.0 / 6.0) * s0 - T(7.0 / 6.0) * s1 + T(11.0 / 6.0) * s2;
-  const T r1 = -T(1.0 / 6.0) * s1 + T(5.0 / 6.0) * s2 + T(2.0 / 6.0) * s3;
-  const T r2 = T(2.0 / 6.0) * s2 + T(5.0 / 6.0) * s3 - T(1.0 / 6.0) * s4;
+#pragma clang fp contract(off)
+  const T c0 = fused(T(-2), s1, s0 + s2), d0 = fused(T(3), s2, fused(T(-4), s1, s0));
+  const T c1 = fused(T(-2), s2, s1 + s3), d1 = s1 - s3;
+  const T c2 = fused(T(-2), s3, s2 + s4), d2 = fused(T(3), s2, fused(T(-4), s3, s4));
+  const T b0 = fused(T(13.0 / 12.0) * c0, c0, T(0.25) * (d0 * d0));
+  const T b1 = fused(T(13.0 / 12.0) * c1, c1, T(0.25) * (d1 * d1));
+  const T b2 = fused(T(13.0 / 12.0) * c2, c2, T(0.25) * (d2 * d2));
+  const T r0 = fused(T(11.0 / 6.0), s2, fused(T(-7.0 / 6.0), s1, T(2.0 / 6.0) * s0));
+  const T r1 = fused(T(2.0 / 6.0), s3, fused(T(5.0 / 6.0), s2, T(-1.0 / 6.0) * s1));
+  const T r2 = fused(T(-1.0 / 6.0), s4, fused(T(5.0 / 6.0), s3, T(2.0 / 6.0) * s2));
   const double v2 = double(v) * double(v);
-  const double e0 = 1e-6 + v2 * double(b0), e1 = 1e-6 + v2 * double(b1), e2 = 1e-6 + v2 * double(b2);
+  const double e0 = fused(v2, double(b0), 1e-6), e1 = fused(v2, double(b1), 1e-6),
+               e2 = fused(v2, double(b2), 1e-6);
   const double e00 = e0 * e0, e11 = e1 * e1, e22 = e2 * e2;
   const double A0 = 0.1 * (e11 * e22), A1 = 0.6 * (e00 * e22), A2 = 0.3 * (e00 * e11);
-  const double num = A0 * double(r0) + A1 * double(r1) + A2 * double(r2);
-  return T(double(v) * (num / (A0 + A1 + A2)));
+  const double num = fused(A2, double(r2), fused(A1, double(r1), A0 * double(r0)));
+  const double den = (A0 + A1) + A2;
+  double r = __builtin_amdgcn_rcp(den);
+  r = fused(fused(-den, r, 1.0), r, r);
+  r = fused(fused(-den, r, 1.0), r, r);
+  double q = num * r;
+  q = fused(fused(-den, q, num), r, q);
+  return T(double(v) * q);
 }
 
 }  // namespace uqlbm

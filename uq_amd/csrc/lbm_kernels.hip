@@ -711,10 +711,29 @@ __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) 
 
 // Face flux (f and h) out of the owner at logical (lx, ly) along axis AX with
 // stencil step sg and speed v = |e_axis|, from the staged direction V: the
-// wall closure at solid owners, else WENO5/NND on the staged line.
+// wall closure at solid owners, else WENO5/NND on the staged line. A tile
+// whose region holds no solid cell (`solids` false) takes the line as
+// staged, on a path of its own (no extrapolation state to carry).
 template <int AX, typename Sh, typename TC>
 __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, const double* alpha, int method,
                                           bool solids, int lx, int ly, int sg, TC v, TC& vf, TC& vh) {
+  const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
+  if (!solids) {
+    if (method == 1) {
+      FH<TC> w[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) w[k] = V[(ly + (k - 2) * dy + kH) * Sh::RX + lx + (k - 2) * dx + kH];
+      vf = weno5(v, w[0].f, w[1].f, w[2].f, w[3].f, w[4].f);
+      vh = weno5(v, w[0].h, w[1].h, w[2].h, w[3].h, w[4].h);
+    } else {
+      FH<TC> w[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) w[k] = V[(ly + (k - 1) * dy + kH) * Sh::RX + lx + (k - 1) * dx + kH];
+      vf = nnd(v, w[0].f, w[1].f, w[2].f);
+      vh = nnd(v, w[0].h, w[1].h, w[2].h);
+    }
+    return;
+  }
   const unsigned mo = R.meta[ly + kH][lx + kH];
   if ((mo & kMetaType) == 2) {
     const TC a = TC(alpha[R.off[ly + kH][lx + kH]]);
@@ -723,7 +742,6 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
     vh = a * v * w.h;
     return;
   }
-  const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
   if (method == 1) {
     TC Vf[5], Vh[5];
     int sld[5], any = 0;
@@ -733,7 +751,7 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
       const FH<TC> w = V[py * Sh::RX + px];
       Vf[k] = w.f;
       Vh[k] = w.h;
-      sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
+      sld[k] = (R.meta[py][px] & kMetaType) == 2;
       any |= sld[k];
     }
     if (any) {
@@ -756,7 +774,7 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
       const FH<TC> w = V[py * Sh::RX + px];
       Vf[k] = w.f;
       Vh[k] = w.h;
-      sld[k] = solids ? (R.meta[py][px] & kMetaType) == 2 : 0;
+      sld[k] = (R.meta[py][px] & kMetaType) == 2;
       any |= sld[k];
     }
     if (any) {
