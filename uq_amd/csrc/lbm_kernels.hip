@@ -276,7 +276,9 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     ruy += f[i] * (lat_y(i) * in.c);
     sh += h[i];
   }
-  const double drho = in.dmom[o], drux = in.dmom[P + o], druy = in.dmom[2 * P + o], dsh = in.dmom[3 * P + o];
+  const double drho = __builtin_nontemporal_load(&in.dmom[o]), drux = __builtin_nontemporal_load(&in.dmom[P + o]),
+               druy = __builtin_nontemporal_load(&in.dmom[2 * P + o]),
+               dsh = __builtin_nontemporal_load(&in.dmom[3 * P + o]);
   Macro m1, m3;
   finish_macro(in, rho, rux, ruy, sh, m1);
   finish_macro(in, rho - drho, rux - drux, ruy - druy, sh - dsh, m3);
