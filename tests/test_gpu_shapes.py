@@ -37,14 +37,18 @@ def _state(lbm, case, shape, precision="fp64", steps=None):
         lbm.set_flux_shape(precision, 0)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("case", DECKS + [_mirrored()], ids=lambda c: f"{c.name}_rk{c.params['rk_method']}")
-def test_shapes_bitwise_equal(gpu_lib, case):
-    print(f"[shapes] {case.name} dims {[gpu_lib.flux_shape_dims(k) for k in range(gpu_lib.FLUX_SHAPES)]}",
-          flush=True)
-    ref = _state(gpu_lib, case, 0)
+def test_shapes_bitwise_equal(gpu_lib, case, precision):
+    """Also the check on the FP32 staging, which streams the next direction
+    into a second LDS buffer while the current one is read: a missing wait
+    would show as a shape-dependent (tile-timing-dependent) state."""
+    print(f"[shapes] {case.name} {precision} dims "
+          f"{[gpu_lib.flux_shape_dims(k) for k in range(gpu_lib.FLUX_SHAPES)]}", flush=True)
+    ref = _state(gpu_lib, case, 0, precision)
     assert ref[0] == 0
     for shape in range(1, gpu_lib.FLUX_SHAPES):
-        st, f, h = _state(gpu_lib, case, shape)
+        st, f, h = _state(gpu_lib, case, shape, precision)
         assert st == 0
         assert np.array_equal(f, ref[1]) and np.array_equal(h, ref[2]), shape
 

@@ -511,10 +511,12 @@ __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef
 // Stencil tiles: region (tile + 3-cell halo cross), staging and faces.
 //
 // A tile shape is TX x TY own cells worked by NT threads. Per direction the
-// threads stage the own cells and the halo bands of the direction's nonzero
-// axes into LDS, compute the (TX+1) x TY x faces and TX x (TY+1) y faces
-// (each face once, by the upwind owner's rule), then each own fluid cell's
-// net flux. The shapes below keep the face and cell counts at or below one
+// threads stage the direction's f, h over the tile and its halo into LDS
+// (FP64 through registers: the own cells and the halo bands of the
+// direction's nonzero axes; FP32 with direct-to-LDS loads of the whole
+// region into one of two buffers, a direction ahead), compute the (TX+1) x
+// TY x faces and TX x (TY+1) y faces (each face once, by the upwind owner's
+// rule), then each own fluid cell's net flux. The shapes below keep the face and cell counts at or below one
 // (or two) per thread so no wave waits on a straggler at the barriers; the
 // autotuner picks one per launch group.
 // ---------------------------------------------------------------------------
@@ -1017,14 +1019,17 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
   }
 }
 
-// FP32 storage: the compiler's own register allocation. FP64: capped at 80
-// registers for 6 waves per SIMD.
+// FP32 storage: capped at 64 registers for 8 waves per SIMD (4 blocks per
+// CU) except with the moment sums (3 spilled slots; mixed-ensemble stage-3
+// flux 3.35 -> 3.18 ms, while the moment-summing stage measured 2 % slower
+// capped and keeps the compiler's allocation). FP64: capped at 80 registers
+// for 6 waves per SIMD.
 // (The staging buffers are separate LDS variables so that the compiler sees
 // that reads of one do not wait for the loads into the other; the second
 // is dropped where it is unused.)
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
-__global__ __launch_bounds__(Sh::NT) void k_flux(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                                 const int* stepp) {
+__global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(MOM ? 2 : 8))) void k_flux(
+    const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, TC, MOM> S;
   __shared__ StageBuf<Sh, TS> VA;
