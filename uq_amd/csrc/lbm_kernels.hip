@@ -671,9 +671,17 @@ __device__ __forceinline__ void setup_slots(SlotSet<Sh>& L, const Region<Sh>& R)
   }
 }
 
+// The staged values of one direction held in registers between their loads
+// and their LDS writes, so that the loads of direction I+1 can be issued
+// while direction I updates its cells.
+template <typename Sh, typename TS>
+struct SlotVals {
+  TS f[Sh::Slots], h[Sh::Slots];
+};
+
 template <typename TS, typename Sh>
-__device__ __forceinline__ void stage_slots(StageBuf<Sh, TS>& B, const SlotSet<Sh>& L, const void* src,
-                                            long long P, int I) {
+__device__ __forceinline__ void fetch_slots(SlotVals<Sh, TS>& V, const SlotSet<Sh>& L, const void* src, long long P,
+                                            int I) {
   const int ex = c_lat_x[I], ey = c_lat_y[I];
   const int dew = c_mir_ew[I], dns = c_mir_ns[I], dboth = c_mir_ns[dew];
 #pragma unroll
@@ -683,7 +691,20 @@ __device__ __forceinline__ void stage_slots(StageBuf<Sh, TS>& B, const SlotSet<S
     const unsigned m = (unsigned)L.meta[j];
     const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
     const long long o = L.off[j];
-    B.v[L.lds[j]] = FH<TS>{plane<TS>(const_cast<void*>(src), P, d)[o], plane<TS>(const_cast<void*>(src), P, kQ + d)[o]};
+    V.f[j] = plane<TS>(const_cast<void*>(src), P, d)[o];
+    V.h[j] = plane<TS>(const_cast<void*>(src), P, kQ + d)[o];
+  }
+}
+
+template <typename TS, typename Sh>
+__device__ __forceinline__ void store_slots(StageBuf<Sh, TS>& B, const SlotVals<Sh, TS>& V, const SlotSet<Sh>& L,
+                                            int I) {
+  const int ex = c_lat_x[I], ey = c_lat_y[I];
+#pragma unroll
+  for (int j = 0; j < Sh::Slots; j++) {
+    const int kind = L.meta[j] >> 8;
+    if (kind == 3 || (kind == 1 && !ex) || (kind == 2 && !ey)) continue;
+    B.v[L.lds[j]] = FH<TS>{V.f[j], V.h[j]};
   }
 }
 
@@ -836,7 +857,7 @@ struct TileWork {
 //   through registers on entry.
 template <typename TS, typename TC, int MODE, bool MOM, bool DB, typename Sh>
 __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
-                                         const TileWork<TS, TC, Sh>& W, const SlotSet<Sh>& L,
+                                         const TileWork<TS, TC, Sh>& W, const SlotSet<Sh>& L, SlotVals<Sh, TS>& PV,
                                          StageBuf<Sh, TS>& cur, StageBuf<Sh, TS>& nxt, int I, bool next) {
   const int t = threadIdx.x;
   const long long P = W.P;
@@ -847,7 +868,15 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   const double* alpha = DB ? W.alpha : in.alpha;
   const int ex = c_lat_x[I], ey = c_lat_y[I];
   const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
-  if constexpr (!DB) stage_slots<TS>(cur, L, W.src, P, I);  // cur was last read before the previous barrier
+  if constexpr (!DB) {  // cur was last read before the previous barrier
+    if constexpr (MOM) {
+      SlotVals<Sh, TS> V;
+      fetch_slots<TS>(V, L, W.src, P, I);
+      store_slots<TS>(cur, V, L, I);
+    } else {
+      store_slots<TS>(cur, PV, L, I);
+    }
+  }
   __syncthreads();  // cur complete; fx, fy (DB: nxt) free
   if (DB && next) stage_dir<TS>(nxt, R, W.src, P, I + 1);
   // faces: x face j of row r (owner column j - [sx > 0]); y face j of
@@ -868,6 +897,23 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   }
   if constexpr (DB) __builtin_amdgcn_s_waitcnt(kWaitVmem);  // this wave's loads into nxt landed
   __syncthreads();
+  // One buffer without the moment sums: the loads of direction I+1 go out
+  // now and land in registers while the cells update (written to LDS when
+  // I+1 starts; stage-3 flux 5-6 % faster, while the moment-summing kernel,
+  // short of registers, loads at the start of each direction instead).
+  // MODE 1 first issues the loads of the values it updates, which it needs
+  // sooner.
+  TS oldf[Sh::Cells], oldh[Sh::Cells];
+  if constexpr (MODE == 1) {
+#pragma unroll
+    for (int q = 0; q < Sh::Cells; q++) {
+      oldf[q] = W.fluid[q] ? W.dst[P * I + W.oo[q]] : TS(0);
+      oldh[q] = W.fluid[q] ? W.dst[P * (kQ + I) + W.oo[q]] : TS(0);
+    }
+  }
+  if constexpr (!DB && !MOM) {
+    if (next) fetch_slots<TS>(PV, L, W.src, P, I + 1);
+  }
   const TC rdx = W.rdx, rdy = W.rdy;
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
@@ -930,10 +976,8 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
         S.mom[3][ci] += double(TS(dH));
       }
     } else {
-      TS* pf = W.dst + P * I + o;
-      TS* ph = W.dst + P * (kQ + I) + o;
-      *pf = TS(TC(*pf) - TC(0.5) * dF);
-      *ph = TS(TC(*ph) - TC(0.5) * dH);
+      W.dst[P * I + o] = TS(TC(oldf[q]) - TC(0.5) * dF);
+      W.dst[P * (kQ + I) + o] = TS(TC(oldh[q]) - TC(0.5) * dH);
     }
   }
 }
@@ -979,16 +1023,18 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
     }
   }
   SlotSet<Sh> L;
+  SlotVals<Sh, TS> PV;
   if constexpr (DB) {
     stage_dir<TS>(VA, R, W.src, W.P, 1);
     __builtin_amdgcn_s_waitcnt(kWaitVmem);
     for (int I = 1; I < kQ; I += 2) {
-      flux_dir<TS, TC, MODE, MOM, true>(S, R, in, W, L, VA, VB, I, true);
-      flux_dir<TS, TC, MODE, MOM, true>(S, R, in, W, L, VB, VA, I + 1, I + 2 < kQ);
+      flux_dir<TS, TC, MODE, MOM, true>(S, R, in, W, L, PV, VA, VB, I, true);
+      flux_dir<TS, TC, MODE, MOM, true>(S, R, in, W, L, PV, VB, VA, I + 1, I + 2 < kQ);
     }
   } else {
     setup_slots(L, R);
-    for (int I = 1; I < kQ; I++) flux_dir<TS, TC, MODE, MOM, false>(S, R, in, W, L, VA, VA, I, false);
+    if constexpr (!MOM) fetch_slots<TS>(PV, L, W.src, W.P, 1);
+    for (int I = 1; I < kQ; I++) flux_dir<TS, TC, MODE, MOM, false>(S, R, in, W, L, PV, VA, VA, I, I + 1 < kQ);
   }
   if (MOM) {
 #pragma unroll
