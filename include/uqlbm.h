@@ -199,7 +199,9 @@ typedef struct lbm_timing {
  * Flux tile shapes and the autotuner. The stencil kernels work on tiles of
  * own cells plus a 3-cell halo cross; shape 0 = 15 x 16 (256 threads),
  * 1 = 31 x 15 (512), 2 = 15 x 32 (512), 3 = 39 x 12 (512), 4 = 23 x 20 (512). lbm_batch_autotune times `steps`
- * steps of the batch with each shape, keeps the fastest for the batch and
+ * steps of the batch with each shape, keeps the fastest (the lowest-numbered
+ * shape within 0.5 % of the fastest, so that near-ties resolve the same way
+ * from run to run) for the batch and
  * as the default of solvers, batches and servers created afterwards with
  * the batch's precision, restores the members' states (tuning changes no
  * result) and reports the shape and each shape's milliseconds per step

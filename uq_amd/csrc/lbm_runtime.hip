@@ -999,8 +999,11 @@ extern "C" int lbm_flux_shape_dims(int32_t shape, int32_t* w, int32_t* h) {
 
 // Tile-shape autotuner: times `steps` steps of the batch with each flux tile
 // shape (after one untimed step that also captures the step graphs), keeps
-// the fastest, and restores every member's state, step count and failure
-// flag, so tuning leaves the solution untouched.
+// the lowest-numbered shape within kTuneTie of the fastest (shapes that close
+// trade places from run to run on timing noise alone; the tie-break makes the
+// choice repeatable), and restores every member's state, step count and
+// failure flag, so tuning leaves the solution untouched.
+static constexpr double kTuneTie = 0.005;
 extern "C" int lbm_batch_autotune(lbm_batch* hb, int32_t steps, int32_t* shape_out, double* ms_per_step_out) {
   if (!hb || steps < 1) return fail_with(LBM_ERR_ARG, "bad autotune arguments");
   Batch& b = *reinterpret_cast<Batch*>(hb);
@@ -1029,8 +1032,7 @@ extern "C" int lbm_batch_autotune(lbm_batch* hb, int32_t steps, int32_t* shape_o
     saved[k].failed = s.failed;
   }
   HIP_TRY(hipStreamSynchronize(b.stream));
-  double best = 0;
-  int best_shape = b.flux_shape;
+  double ms_of[kFluxShapes];
   for (int shape = 0; shape < kFluxShapes; shape++) {
     int st = batch_set_shape(b, shape);
     if (st == LBM_OK) st = batch_step(b, 1);
@@ -1041,11 +1043,12 @@ extern "C" int lbm_batch_autotune(lbm_batch* hb, int32_t steps, int32_t* shape_o
     }
     const double ms = g_last_timing.total_ms / steps;
     if (ms_per_step_out) ms_per_step_out[shape] = ms;
-    if (shape == 0 || ms < best) {
-      best = ms;
-      best_shape = shape;
-    }
+    ms_of[shape] = ms;
   }
+  double fastest = ms_of[0];
+  for (int shape = 1; shape < kFluxShapes; shape++) fastest = std::min(fastest, ms_of[shape]);
+  int best_shape = 0;
+  while (ms_of[best_shape] > fastest * (1.0 + kTuneTie)) best_shape++;
   for (size_t k = 0; k < b.members.size(); k++) {
     Solver& s = *b.members[k];
     const size_t bytes = (size_t)s.pitch * s.ny * kPlanes * es;
