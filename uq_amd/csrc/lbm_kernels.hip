@@ -853,8 +853,10 @@ struct TileWork {
 // Direction I of a tile.
 //   DB (two staging buffers): faces from cur, landed by the barrier on
 //   entry, while the loads of direction I+1 (`next`) stream into nxt.
-//   One buffer (cur and nxt the same): the threads stage direction I
-//   through registers on entry.
+//   One buffer (cur and nxt the same): direction I's values go from
+//   registers to cur on entry; they were loaded into PV while direction
+//   I-1 updated its cells (with the moment sums: loaded here, on entry),
+//   and PV takes the loads of direction I+1 once the faces are done.
 template <typename TS, typename TC, int MODE, bool MOM, bool DB, typename Sh>
 __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region<Sh>& R, const InstDev& in,
                                          const TileWork<TS, TC, Sh>& W, const SlotSet<Sh>& L, SlotVals<Sh, TS>& PV,
@@ -983,7 +985,7 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
 }
 
 // Staging per storage type, as measured on the two workloads (4096^2
-// Riemann, 256-member Couette ensemble): FP32 two buffers with loads straight
+// Riemann, the 256-member UQ ensemble): FP32 two buffers with loads straight
 // into LDS (stage-3 flux 3.85 -> 3.56 ms on the ensemble); FP64 one buffer
 // through registers, which is as fast or faster there (4.75-4.77 against
 // 4.87 ms) and leaves room for three 8-wave blocks per CU with the moment
