@@ -100,6 +100,8 @@ def test_autotune_keeps_state_and_continues_identically(gpu_lib, precision):
         shape, ms = ba.autotune(steps=2)
         print(f"[shapes] chosen {shape} ms/step {ms}", flush=True)
         assert 0 <= shape < gpu_lib.FLUX_SHAPES and all(m > 0 for m in ms)
+        # the lowest-numbered shape within 0.5 % of the fastest
+        assert shape == min(k for k in range(gpu_lib.FLUX_SHAPES) if ms[k] <= min(ms) * (1.0 + 0.005))
         for s, (f0, h0) in zip(a, before):
             f1, h1 = s.get_dist()
             assert np.array_equal(f0, f1) and np.array_equal(h0, h1)

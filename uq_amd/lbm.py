@@ -219,7 +219,8 @@ class Batch:
         return _check(lib().lbm_batch_step(self._h, int(n)), "lbm_batch_step", allow_failed=True)
 
     def autotune(self, steps: int = 5) -> tuple[int, list[float]]:
-        """Pick the fastest flux tile shape for this batch (state unchanged);
+        """Pick the fastest flux tile shape for this batch (the lowest-numbered
+        shape within 0.5 % of the fastest; state unchanged);
         returns the shape and each shape's milliseconds per step."""
         shape = C.c_int32(0)
         ms = np.zeros(FLUX_SHAPES)
