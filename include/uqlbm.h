@@ -157,6 +157,18 @@ int lbm_set_dist(lbm_solver* s, const double* f, const double* h);
  * T is the cell temperature in deck units. */
 int lbm_set_viscosity(lbm_solver* s, int32_t model, double mu_ref, double T_ref, double param);
 
+/* Non-uniform grid of the reference's OpenCL front end
+ * (CODE/Python/simLB/codeCL.py:611-618, 676-679): dx[nx] per column and
+ * dy[ny] per row. Face fluxes are weighted by their face length and the net
+ * flux divided by the cell area, i.e. the cell's own dt/dx[x] and dt/dy[y]
+ * in cFLUX; dt is taken from the smallest spacing
+ * (CODE/Python/simLB/LBM.py:216) and the step count re-derived. Before the
+ * first step (LBM_ERR_STATE after); LBM_ERR_ARG for non-positive spacings
+ * and for decks with solid cells (the OpenCL wall closure is not the C++
+ * one this library follows). Totals (lbm_get_totals) stay unweighted cell
+ * sums; the conserved quantities are the sums weighted by dx[x] dy[y]. */
+int lbm_set_spacing(lbm_solver* s, const double* dx, const double* dy);
+
 /* Conserved totals over the fluid cells (cellType 0), FP64 sums:
  * out[0] = sum f (mass), out[1] = sum f e_x, out[2] = sum f e_y (momentum),
  * out[3] = sum h (total energy), e in physical units (lattice x c). The

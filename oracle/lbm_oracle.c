@@ -68,6 +68,10 @@ struct orc {
    * 2 Sutherland mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S) with S = param */
   int visc_model;
   double visc_mu_ref, visc_T_ref, visc_param;
+
+  /* grid spacing per column and per row (uniform: the deck's dx, dy; the
+   * OpenCL front end's non-uniform grids, PY/codeCL.py:611-618, 676-679) */
+  double *dxs, *dys;
 };
 
 #define CELL(o, x, y) ((size_t)(x) * (size_t)(o)->ny + (size_t)(y))
@@ -403,7 +407,7 @@ static double net_flux(const orc* o, const double* flux, int x, int y, int i) {
   const int xu = map_index(o, x, -sx, 0, &mir_y);
   const int yu = map_index(o, y, -sy, 1, &mir_x);
   mirror_dirs(i, mir_x, mir_y, &i_x, &i_y);
-  const double rx = o->dt / o->p.dx, ry = o->dt / o->p.dy;
+  const double rx = o->dt / o->dxs[x], ry = o->dt / o->dys[y];
   const double out = rx * flux[2 * DIST(o, x, y, i)] + ry * flux[2 * DIST(o, x, y, i) + 1];
   const double in = rx * flux[2 * DIST(o, xu, y, i_y)] + ry * flux[2 * DIST(o, x, yu, i_x) + 1];
   return out - in;
@@ -752,6 +756,8 @@ static void refine_input(orc* o) {
 }
 
 /* CPP/cpuLBM.cpp:360-450 */
+static void steps_from_dt(orc* o);
+
 static void setup_vars(orc* o) {
   static const int lat_x[NQ] = {0, 1, 0, -1, 0, 1, -1, -1, 1, 2, 0, -2, 0};
   static const int lat_y[NQ] = {0, 0, 1, 0, -1, 1, 1, -1, -1, 0, 2, 0, -2};
@@ -769,7 +775,12 @@ static void setup_vars(orc* o) {
   }
   if (p->dtau == 0) o->dt = (p->cfl * (p->dx < p->dy ? p->dx : p->dy)) / o->ex[9];
   else o->dt = p->dtau * o->tau_ref;
+  steps_from_dt(o);
+}
 
+/* nsteps from tt_tref, tt_time or steps (CPP/cpuLBM.cpp:423-448) */
+static void steps_from_dt(orc* o) {
+  const lbm_params* p = &o->p;
   double total = 0;
   int need = 0;
   o->nsteps = 0;
@@ -836,6 +847,9 @@ orc* orc_create(const lbm_params* p, int32_t nx, int32_t ny, const int32_t* bnd,
   o->dF2 = dalloc(nq); o->dH2 = dalloc(nq);
   refine_input(o);
   setup_vars(o);
+  o->dxs = dalloc(nx); o->dys = dalloc(ny);
+  for (int x = 0; x < nx; x++) o->dxs[x] = o->p.dx;
+  for (int y = 0; y < ny; y++) o->dys[y] = o->p.dy;
   init_functions(o);
   o->step = 0;
   o->tstep = 0;
@@ -849,7 +863,7 @@ void orc_destroy(orc* o) {
                      &o->hr3, &o->feq1, &o->heq1, &o->feq3, &o->heq3, &o->rho1, &o->ux1, &o->uy1, &o->p1,
                      &o->T1, &o->tf1, &o->th1, &o->thf1, &o->rho3, &o->ux3, &o->uy3, &o->p3, &o->T3,
                      &o->tf3, &o->th3, &o->thf3, &o->flx_f2, &o->flx_h2, &o->flx_f3, &o->flx_h3,
-                     &o->dF2, &o->dH2};
+                     &o->dF2, &o->dH2, &o->dxs, &o->dys};
   for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++) free(*bufs[k]);
   free(o->bnd);
   free(o->ctype);
@@ -951,5 +965,37 @@ int orc_set_viscosity(orc* o, int32_t model, double mu_ref, double T_ref, double
   o->visc_mu_ref = mu_ref;
   o->visc_T_ref = T_ref;
   o->visc_param = param;
+  return 0;
+}
+
+/* Non-uniform spacing, after the OpenCL front end: face fluxes weighted by
+ * their face length (dy[y] for x faces, dx[x] for y faces) and the net
+ * divided by the cell area (PY/codeCL.py:611-618, 676-679). The two cells
+ * of a face share its length, so cFLUX becomes dt/dx[x] and dt/dy[y] of the
+ * cell; dt comes from the smallest spacing (PY/LBM.py:216). Refused (-1) for
+ * non-positive spacings and for decks with solid cells, whose OpenCL wall
+ * closure differs from the C++ one this oracle follows. */
+int orc_set_spacing(orc* o, const double* dx, const double* dy) {
+  if (!o || !dx || !dy) return -1;
+  for (int x = 0; x < o->nx; x++)
+    if (!(dx[x] > 0) || !isfinite(dx[x])) return -1;
+  for (int y = 0; y < o->ny; y++)
+    if (!(dy[y] > 0) || !isfinite(dy[y])) return -1;
+  for (int x = 0; x < o->nx; x++)
+    for (int y = 0; y < o->ny; y++)
+      if (cell_type_at(o, x, y) == 2) return -1;
+  double mn = dx[0];
+  for (int x = 0; x < o->nx; x++) {
+    o->dxs[x] = dx[x];
+    if (dx[x] < mn) mn = dx[x];
+  }
+  for (int y = 0; y < o->ny; y++) {
+    o->dys[y] = dy[y];
+    if (dy[y] < mn) mn = dy[y];
+  }
+  if (o->p.dtau == 0) {
+    o->dt = (o->p.cfl * mn) / o->ex[9];
+    steps_from_dt(o);
+  }
   return 0;
 }

@@ -61,6 +61,10 @@ double orc_cflux(const orc* o, const double* flux, int32_t x, int32_t y,
  * mu_ref (T/T_ref)^(0.5+param), 2 Sutherland mu_ref (T/T_ref)^1.5
  * (T_ref+S)/(T+S), S = param. Returns -1 on bad arguments. */
 int orc_set_viscosity(orc* o, int32_t model, double mu_ref, double T_ref, double param);
+/* Per-column dx[nx] and per-row dy[ny] (PY/codeCL.py:611-618, 676-679):
+ * cFLUX with the cell's own dt/dx, dt/dy; dt from the smallest spacing.
+ * Before the first step. Returns -1 on bad spacings or with solid cells. */
+int orc_set_spacing(orc* o, const double* dx, const double* dy);
 /* Seconds spent inside orc_step (steady clock around the step loop only). */
 double orc_step_seconds(const orc* o);
 

@@ -61,6 +61,7 @@ def lib():
         L.orc_step_seconds.restype = d
         L.orc_step_seconds.argtypes = [vp]
         L.orc_set_viscosity.argtypes = [vp, i32, d, d, d]
+        L.orc_set_spacing.argtypes = [vp, P(d), P(d)]
         _lib = L
     return _lib
 
@@ -99,6 +100,9 @@ class Oracle:
         v = case.meta.get("viscosity") if isinstance(case.meta, dict) else None
         if v:
             self.set_viscosity(v["model"], v["mu_ref"], v["T_ref"], v.get("param", 0.0))
+        sp = case.meta.get("spacing") if isinstance(case.meta, dict) else None
+        if sp:
+            self.set_spacing(sp["dx"], sp["dy"])
 
     def set_viscosity(self, model, mu_ref: float, T_ref: float, param: float = 0.0) -> None:
         """Viscosity law: "constant", "vhs" (exponent 0.5 + param) or
@@ -106,6 +110,16 @@ class Oracle:
         code = {"constant": 0, "vhs": 1, "sutherland": 2}[model] if isinstance(model, str) else int(model)
         if lib().orc_set_viscosity(self._h, code, float(mu_ref), float(T_ref), float(param)) != 0:
             raise ValueError("bad viscosity law")
+
+    def set_spacing(self, dx, dy) -> None:
+        """Per-column dx[nx] and per-row dy[ny] (non-uniform grid of the
+        OpenCL front end); before the first step; no solid cells."""
+        self._dx = np.ascontiguousarray(dx, dtype=np.float64)
+        self._dy = np.ascontiguousarray(dy, dtype=np.float64)
+        if self._dx.shape != (self.nx,) or self._dy.shape != (self.ny,):
+            raise ValueError("spacing arrays must have nx and ny entries")
+        if lib().orc_set_spacing(self._h, dptr(self._dx), dptr(self._dy)) != 0:
+            raise ValueError("bad spacing (non-positive, or a deck with solid cells)")
 
     def close(self):
         if self._h:

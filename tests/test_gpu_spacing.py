@@ -1,0 +1,128 @@
+"""Non-uniform grids (lbm_set_spacing; the OpenCL front end's per-column dx
+and per-row dy, PY/codeCL.py:611-618, 676-679) on the GPU: uniform arrays
+reproduce the uniform path bit for bit; stretched grids match the oracle
+within max(1e-12, 8 x the oracle's one-ulp sensitivity) relative L2 (a
+doubly periodic deck whose state varies along both axes, and the clamped
+Riemann deck); a batch mixing spaced and uniform
+members gives each member's result alone; decks with solid cells and late
+calls are refused."""
+import numpy as np
+import pytest
+
+from uq_amd import cases
+from uq_amd.lbm import LbmError, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _modulate(get_dist, set_dist, nx, ny):
+    f, h = get_dist()
+    m = 1 + 0.02 * np.outer(np.sin(2 * np.pi * np.arange(nx) / nx), np.cos(2 * np.pi * np.arange(ny) / ny))
+    set_dist(f * m[:, :, None], h * m[:, :, None])
+
+
+def _gpu(lbm, case, steps, precision="fp64", modulate=False):
+    s = lbm.Solver(case, precision)
+    try:
+        if modulate:
+            _modulate(s.get_dist, s.set_dist, case.nx, case.ny)
+        st = s.step(steps)
+        f, h = s.get_dist()
+        return st, f, h
+    finally:
+        s.close()
+
+
+def _oracle(case, steps, modulate=False, perturb=False):
+    from pyoracle import Oracle
+    o = Oracle(case)
+    try:
+        if modulate:
+            _modulate(o.get_dist, o.set_dist, case.nx, case.ny)
+        if perturb:  # one ulp, random signs: the oracle's own sensitivity
+            f, h = o.get_dist()
+            rng = np.random.default_rng(1)
+            eps = np.finfo(np.float64).eps
+            o.set_dist(f * (1 + eps * rng.choice([-1.0, 1.0], f.shape)),
+                       h * (1 + eps * rng.choice([-1.0, 1.0], h.shape)))
+        st = o.step(steps)
+        f, h = o.get_dist()
+        return st, f, h
+    finally:
+        o.close()
+
+
+def test_uniform_arrays_are_the_uniform_path(gpu_lib):
+    c = cases.shear_wave(40, steps=12)
+    ref = _gpu(gpu_lib, c, 12)
+    got = _gpu(gpu_lib, cases.with_spacing(c, np.full(c.nx, c.params["dx"]), np.full(c.ny, c.params["dy"])), 12)
+    assert ref[0] == 0 and got[0] == 0
+    assert np.array_equal(ref[1], got[1]) and np.array_equal(ref[2], got[2])
+
+
+@pytest.mark.parametrize("deck", ["periodic", "riemann"])
+def test_stretched_grid_matches_oracle(gpu_lib, deck):
+    if deck == "periodic":
+        c, steps, mod = cases.stretched(cases.shear_wave(48, u0=0.15, steps=30), amp=0.3), 30, True
+    else:
+        c, steps, mod = cases.stretched(cases.riemann(64, steps=6), amp=0.2), 6, False
+    st_o, fo, ho = _oracle(c, steps, mod)
+    st_g, fg, hg = _gpu(gpu_lib, c, steps, modulate=mod)
+    ef, eh = rel_l2(fg, fo), rel_l2(hg, ho)
+    # bar: max(1e-12, 8 x the oracle's one-ulp sensitivity), as for the
+    # golden fixtures (the Riemann shocks amplify rounding)
+    _, fp, hp = _oracle(c, steps, mod, perturb=True)
+    bar = max(1e-12, 8.0 * max(rel_l2(fp, fo), rel_l2(hp, ho)))
+    print(f"[spacing] {c.name} {steps} steps: f {ef:.2e} h {eh:.2e} bar {bar:.2e}", flush=True)
+    assert st_o == 0 and st_g == 0
+    assert ef <= bar and eh <= bar
+    # the spacing matters: the uniform deck ends elsewhere
+    st_u, fu, _ = _gpu(gpu_lib, cases.shear_wave(48, u0=0.15, steps=30) if deck == "periodic"
+                       else cases.riemann(64, steps=6), steps, modulate=mod)
+    assert rel_l2(fu, fo) > 1e-6
+
+
+def test_batch_mixing_spaced_and_uniform_members(gpu_lib):
+    a = cases.stretched(cases.shear_wave(32, u0=0.12, steps=20), amp=0.25)
+    b = cases.shear_wave(48, u0=0.1, steps=20)
+    alone = [_gpu(gpu_lib, c, 20) for c in (a, b)]
+    ss = [gpu_lib.Solver(c, "fp64") for c in (a, b)]
+    bt = gpu_lib.Batch(ss)
+    try:
+        assert bt.step(20) == 0
+        for s, (st, f, h) in zip(ss, alone):
+            fb, hb = s.get_dist()
+            assert np.array_equal(fb, f) and np.array_equal(hb, h)
+    finally:
+        bt.close()
+        for s in ss:
+            s.close()
+
+
+def test_mixed_precision_on_a_stretched_grid(gpu_lib):
+    c = cases.stretched(cases.shear_wave(48, u0=0.15, steps=30), amp=0.3)
+    st64, f64, _ = _gpu(gpu_lib, c, 30)
+    st32, f32, _ = _gpu(gpu_lib, c, 30, precision="mixed")
+    e = rel_l2(f32, f64)
+    print(f"[spacing] mixed vs fp64: {e:.2e}", flush=True)
+    assert st64 == 0 and st32 == 0 and e <= 1e-6
+
+
+def test_refusals(gpu_lib):
+    wall = cases.couette(24, steps=5)
+    s = gpu_lib.Solver(wall, "fp64")
+    try:
+        with pytest.raises(LbmError):
+            s.set_spacing(np.full(wall.nx, wall.params["dx"]), np.full(wall.ny, wall.params["dy"]))
+    finally:
+        s.close()
+    c = cases.shear_wave(16, steps=5)
+    s = gpu_lib.Solver(c, "fp64")
+    try:
+        with pytest.raises(LbmError):
+            s.set_spacing(np.full(c.nx, c.params["dx"]), -np.ones(c.ny))
+        s.step(1)
+        with pytest.raises(LbmError):
+            s.set_spacing(np.full(c.nx, c.params["dx"]), np.full(c.ny, c.params["dy"]))
+    finally:
+        s.close()

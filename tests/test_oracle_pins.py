@@ -406,3 +406,82 @@ def test_couette_steady_profile_matches_slip_solution():
     assert slope == pytest.approx(prm["A"], rel=0.05)
     # the slip length is visible: the profile does not reach the wall speeds
     assert u_sim[0] > 0.2 * prm["A"] / (c.ny - 2)
+
+
+# --------------------------------------------------------------------------
+# Non-uniform grid of the OpenCL front end (PY/codeCL.py:611-618, 676-679):
+# face fluxes weighted by face length, net divided by cell area. The
+# finite-volume form telescopes, so on a doubly periodic grid the sums
+# weighted by the cell areas dx[x] dy[y] are conserved to round-off; a wrong
+# index (the upwind cell's spacing, a swapped dx/dy) breaks that.
+# --------------------------------------------------------------------------
+def _weighted_totals(o, dx, dy):
+    f, h = o.get_dist()
+    vol = np.outer(dx, dy)
+    return np.array([(vol * f.sum(axis=2)).sum(), (vol * (f * EX).sum(axis=2)).sum(),
+                     (vol * (f * EY).sum(axis=2)).sum(), (vol * h.sum(axis=2)).sum()])
+
+
+def test_spacing_uniform_arrays_are_the_scalar_path():
+    c = cases.shear_wave(24, steps=10)
+    a = Oracle(c)
+    a.step(10)
+    fa, ha = a.get_dist()
+    a.close()
+    b = Oracle(cases.with_spacing(c, np.full(c.nx, c.params["dx"]), np.full(c.ny, c.params["dy"])))
+    b.step(10)
+    fb, hb = b.get_dist()
+    b.close()
+    assert np.array_equal(fa, fb) and np.array_equal(ha, hb)
+
+
+def _modulate(o, nx, ny):
+    """Make the state vary along both axes (the shear wave varies along y
+    only, and an x-invariant field has zero x-flux differences)."""
+    f, h = o.get_dist()
+    m = 1 + 0.02 * np.outer(np.sin(2 * np.pi * np.arange(nx) / nx), np.cos(2 * np.pi * np.arange(ny) / ny))
+    o.set_dist(f * m[:, :, None], h * m[:, :, None])
+
+
+def test_spacing_conserves_area_weighted_totals():
+    c = cases.stretched(cases.shear_wave(32, u0=0.15, steps=40), amp=0.3)
+    dx = np.array(c.meta["spacing"]["dx"])
+    dy = np.array(c.meta["spacing"]["dy"])
+    o = Oracle(c)
+    _modulate(o, c.nx, c.ny)
+    t0 = _weighted_totals(o, dx, dy)
+    o.step(40)
+    t1 = _weighted_totals(o, dx, dy)
+    f_spaced, _ = o.get_dist()
+    o.close()
+    scale = np.array([t0[0], t0[0], t0[0], t0[3]])
+    drift = np.abs(t1 - t0) / scale
+    assert np.all(drift <= 1e-13), drift
+    # and the spacing changes the flow (the unweighted mass is not conserved)
+    u = Oracle(cases.shear_wave(32, u0=0.15, steps=40))
+    _modulate(u, c.nx, c.ny)
+    u.step(40)
+    f_uniform, _ = u.get_dist()
+    u.close()
+    assert np.abs(f_spaced - f_uniform).max() > 1e-6
+    w = f_spaced.sum()
+    assert abs(w - Oracle(c).get_dist()[0].sum()) / w > 1e-12
+
+
+def test_spacing_time_step_and_refusals():
+    c = cases.shear_wave(16, steps=5)
+    dx = np.full(c.nx, c.params["dx"])
+    dy = np.full(c.ny, c.params["dy"])
+    dy[3] = 0.25 * c.params["dy"]
+    o = Oracle(c)
+    dt0 = o.consts()["dt"]
+    o.set_spacing(dx, dy)
+    assert o.consts()["dt"] == pytest.approx(0.25 * dt0, rel=1e-15)   # dt from the smallest spacing
+    with pytest.raises(ValueError):
+        o.set_spacing(dx, -dy)
+    o.close()
+    wall = cases.couette(16, steps=5)                                  # solid rows: refused
+    ow = Oracle(wall)
+    with pytest.raises(ValueError):
+        ow.set_spacing(np.full(wall.nx, wall.params["dx"]), np.full(wall.ny, wall.params["dy"]))
+    ow.close()

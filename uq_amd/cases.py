@@ -313,6 +313,30 @@ def request_stream(rate: float = 50.0, duration: float = 60.0, seed: int = 7,
     return out
 
 
+def with_spacing(case: Case, dx, dy) -> Case:
+    """The case on a non-uniform grid (the OpenCL front end's per-column dx
+    and per-row dy, PY/codeCL.py:611-618, 676-679): applied by Solver and the
+    oracle at construction. Decks with solid cells are refused there."""
+    dx = [float(v) for v in dx]
+    dy = [float(v) for v in dy]
+    if len(dx) != case.nx or len(dy) != case.ny:
+        raise ValueError("spacing arrays must have nx and ny entries")
+    meta = dict(case.meta)
+    meta["spacing"] = {"dx": dx, "dy": dy}
+    return Case(case.name + "_spaced", dict(case.params), case.bnd, case.props, case.nsteps, meta)
+
+
+def stretched(case: Case, amp: float = 0.3) -> Case:
+    """A smoothly stretched grid for `case`: dx[x] = dx (1 + amp sin(2 pi
+    x / nx)), dy[y] = dy (1 + amp cos(2 pi y / ny)) (periodic in both
+    directions, so a doubly periodic deck stays periodic)."""
+    x = np.arange(case.nx)
+    y = np.arange(case.ny)
+    dx = case.params["dx"] * (1 + amp * np.sin(2 * np.pi * x / case.nx))
+    dy = case.params["dy"] * (1 + amp * np.cos(2 * np.pi * y / case.ny))
+    return with_spacing(case, dx, dy)
+
+
 def couette_analytic(case: Case, y_cells: np.ndarray, c_p: float | None = None):
     """Steady thermal Couette profile with first-order Knudsen slip and
     temperature jump, after the reference's own check

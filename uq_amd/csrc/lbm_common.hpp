@@ -59,6 +59,10 @@ struct InstDev {
   // mu_ref (T/T_ref)^1.5 (T_ref + vS)/(T + vS)
   int visc_model;
   double mu_ref, T_ref_v, iT_ref_v, vexp, vS;
+  // non-uniform grid: dt/dx per column [nx] and dt/dy per row [ny]; null
+  // for a uniform grid (rdx, rdy above)
+  const double* rdxs;
+  const double* rdys;
 };
 
 // One tile of one instance: the unit of work of a block.

@@ -545,6 +545,8 @@ template <typename Sh>
 struct Region {
   int off[Sh::RY][Sh::RX];
   unsigned char meta[Sh::RY][Sh::RX];
+  double rdx[Sh::TX], rdy[Sh::TY];  // dt/dx of the tile's columns, dt/dy of its rows (non-uniform grids)
+  float rdxf[Sh::TX], rdyf[Sh::TY];  // the same in FP32, for the FP32 stencil arithmetic
 };
 
 template <typename Sh>
@@ -571,6 +573,17 @@ __device__ void setup_region(Region<Sh>& R, const InstDev& in, int x0, int y0) {
     m |= cx ? kMetaClampX : 0u;
     m |= cy ? kMetaClampY : 0u;
     R.meta[ly + kH][lx + kH] = (unsigned char)m;
+  }
+  for (int k = threadIdx.x; k < Sh::TX + Sh::TY; k += Sh::NT) {  // dt/dx, dt/dy of the tile's columns and rows
+    if (k < Sh::TX) {
+      const double v = !in.rdxs ? in.rdx : (x0 + k < in.nx ? in.rdxs[x0 + k] : 0.0);
+      R.rdx[k] = v;
+      R.rdxf[k] = float(v);
+    } else {
+      const double v = !in.rdys ? in.rdy : (y0 + k - Sh::TX < in.ny ? in.rdys[y0 + k - Sh::TX] : 0.0);
+      R.rdy[k - Sh::TX] = v;
+      R.rdyf[k - Sh::TX] = float(v);
+    }
   }
 }
 
@@ -916,14 +929,22 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   if constexpr (!DB && !MOM) {
     if (next) fetch_slots<TS>(PV, L, W.src, P, I + 1);
   }
-  const TC rdx = W.rdx, rdy = W.rdy;
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
     if (!W.fluid[q]) continue;
     const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
     // reference cFLUX (cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) +
     // (dt/dy)(Fy_out - Fy_in), grouped out - in; an upwind owner across a
-    // clamped edge is the cell itself
+    // clamped edge is the cell itself. On a non-uniform grid dt/dx and
+    // dt/dy are the cell's column and row values.
+    TC rdx, rdy;
+    if constexpr (sizeof(TC) == 8) {
+      rdx = TC(R.rdx[tx]);
+      rdy = TC(R.rdy[ty]);
+    } else {
+      rdx = TC(R.rdxf[tx]);
+      rdy = TC(R.rdyf[ty]);
+    }
     TC oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
     if (ex) {
       const int lo = sx > 0 ? -1 : 0;

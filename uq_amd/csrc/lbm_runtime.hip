@@ -50,6 +50,8 @@ static thread_local lbm_timing g_last_timing = {};
 // ---------------------------------------------------------------------------
 // deck refinement (reference refineInput / setupVars, cpuLBM.cpp:283-450)
 // ---------------------------------------------------------------------------
+static void steps_from_dt(Solver& s);
+
 static void refine_deck(Solver& s) {
   lbm_params& p = s.p;
   if (p.dx == 0) p.dx = p.Lx / ((double)s.nx - 1);
@@ -69,11 +71,16 @@ static void refine_deck(Solver& s) {
   s.K = s.b - 2;
   s.Tc = p.Tc_Tref * p.T_ref;
   const double tau_ref = p.mu / p.p_ref;
-  const double u_ref = std::sqrt(p.R * p.T_ref);
-  const double t_ref = std::max(p.Lx, p.Ly) / u_ref;
   s.c = std::sqrt(p.R * s.Tc);
   const double e9 = 2 * s.c;
   s.dt = p.dtau == 0 ? (p.cfl * std::min(p.dx, p.dy)) / e9 : p.dtau * tau_ref;
+  steps_from_dt(s);
+}
+
+// nsteps from tt_tref, tt_time or steps (cpuLBM.cpp:423-448)
+static void steps_from_dt(Solver& s) {
+  const lbm_params& p = s.p;
+  const double t_ref = std::max(p.Lx, p.Ly) / std::sqrt(p.R * p.T_ref);
   double total = 0;
   bool need = false;
   s.nsteps = 0;
@@ -294,7 +301,7 @@ Solver::~Solver() {
   if (graphs) graphs->clear();
   for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
                   (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
-                  (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev})
+                  (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -325,6 +332,8 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.dy = s.p.dy;
   d.rdx = s.dt / s.p.dx;
   d.rdy = s.dt / s.p.dy;
+  d.rdxs = s.spacing_dev;
+  d.rdys = s.spacing_dev ? s.spacing_dev + s.nx : nullptr;
   d.c = s.c;
   d.sRTc = std::sqrt(s.p.R * s.Tc);
   d.Tc = s.Tc;
@@ -863,6 +872,39 @@ extern "C" int lbm_set_viscosity(lbm_solver* h, int32_t model, double mu_ref, do
   s.mu_ref = mu_ref;
   s.T_ref_v = T_ref;
   s.visc_param = param;
+  return upload_inst(s);
+}
+
+// Non-uniform grid spacing after the OpenCL front end (PY/codeCL.py:611-618,
+// 676-679): face fluxes weighted by their face length and the net divided by
+// the cell area, i.e. cFLUX with the cell's own dt/dx[x] and dt/dy[y] (the
+// two cells of a face share its length); dt from the smallest spacing
+// (PY/LBM.py:216), nsteps recomputed. Before the first step; refused for
+// decks with solid cells (the OpenCL wall closure is not the C++ one this
+// path follows).
+extern "C" int lbm_set_spacing(lbm_solver* h, const double* dx, const double* dy) {
+  if (!h || !dx || !dy) return fail_with(LBM_ERR_ARG, "null argument");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (s.step != 0) return fail_with(LBM_ERR_STATE, "spacing must be set before the first step");
+  for (int x = 0; x < s.nx; x++)
+    if (!(dx[x] > 0) || !std::isfinite(dx[x])) return fail_with(LBM_ERR_ARG, "dx[%d] is not positive", x);
+  for (int y = 0; y < s.ny; y++)
+    if (!(dy[y] > 0) || !std::isfinite(dy[y])) return fail_with(LBM_ERR_ARG, "dy[%d] is not positive", y);
+  for (uint8_t t : s.ctype_host)
+    if (t == 2) return fail_with(LBM_ERR_ARG, "non-uniform spacing is not supported with solid cells");
+  s.dxs.assign(dx, dx + s.nx);
+  s.dys.assign(dy, dy + s.ny);
+  const double mn = std::min(*std::min_element(s.dxs.begin(), s.dxs.end()),
+                             *std::min_element(s.dys.begin(), s.dys.end()));
+  if (s.p.dtau == 0) {
+    s.dt = (s.p.cfl * mn) / (2 * s.c);
+    steps_from_dt(s);
+  }
+  std::vector<double> r(s.nx + s.ny);
+  for (int x = 0; x < s.nx; x++) r[x] = s.dt / s.dxs[x];
+  for (int y = 0; y < s.ny; y++) r[s.nx + y] = s.dt / s.dys[y];
+  if (!s.spacing_dev) HIP_TRY(hipMalloc((void**)&s.spacing_dev, r.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(s.spacing_dev, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
   return upload_inst(s);
 }
 

@@ -73,6 +73,8 @@ struct Solver {
   int flux_shape = 0;               // flux tile shape of tiles_dev
   int visc_model = 0;               // viscosity law (lbm_set_viscosity)
   double mu_ref = 0, T_ref_v = 0, visc_param = 0;
+  std::vector<double> dxs, dys;     // non-uniform spacing (lbm_set_spacing), empty when uniform
+  double* spacing_dev = nullptr;    // dt/dx[nx] then dt/dy[ny]
   double* totals_dev = nullptr;     // conserved-totals reduction scratch (allocated on first use)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;

@@ -62,6 +62,7 @@ def lib():
     L.lbm_get_totals.argtypes = [vp, P(d)]
     L.lbm_set_flux_shape.argtypes = [i32, i32]
     L.lbm_set_viscosity.argtypes = [vp, i32, d, d, d]
+    L.lbm_set_spacing.argtypes = [vp, P(d), P(d)]
     L.lbm_flux_shape_dims.argtypes = [i32, P(i32), P(i32)]
     L.lbm_batch_autotune.argtypes = [vp, i32, P(i32), P(d)]
     L.lbm_get_timing.argtypes = [P(LbmTiming)]
@@ -141,8 +142,20 @@ class Solver:
         v = case.meta.get("viscosity") if isinstance(case.meta, dict) else None
         if v:
             self.set_viscosity(v["model"], v["mu_ref"], v["T_ref"], v.get("param", 0.0))
+        sp = case.meta.get("spacing") if isinstance(case.meta, dict) else None
+        if sp:
+            self.set_spacing(sp["dx"], sp["dy"])
         if init:
             self.init()
+
+    def set_spacing(self, dx, dy) -> None:
+        """Non-uniform grid: dx per column (nx) and dy per row (ny), before
+        the first step; decks with solid cells are refused."""
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        dy = np.ascontiguousarray(dy, dtype=np.float64)
+        if dx.shape != (self.nx,) or dy.shape != (self.ny,):
+            raise ValueError("spacing arrays must have nx and ny entries")
+        _check(lib().lbm_set_spacing(self._h, dptr(dx), dptr(dy)), "lbm_set_spacing")
 
     def set_viscosity(self, model, mu_ref: float, T_ref: float, param: float = 0.0) -> None:
         """Viscosity law: "constant", "vhs" (exponent 0.5 + param) or
