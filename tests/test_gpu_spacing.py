@@ -82,6 +82,21 @@ def test_stretched_grid_matches_oracle(gpu_lib, deck):
     assert rel_l2(fu, fo) > 1e-6
 
 
+@pytest.mark.parametrize("rk,fm", [(0, 1), (2, 1), (3, 1), (1, 0)], ids=["rk1", "euler", "euler_eq", "rk3_nnd"])
+def test_stretched_grid_other_steppers_and_nnd(gpu_lib, rk, fm):
+    """The spacing enters the net flux that every stepper and both
+    reconstructions share (RKMETHOD 0/2/3, FMETHOD 0)."""
+    c = cases.stretched(cases.shear_wave(40, u0=0.12, steps=20, rk_method=rk, flux_method=fm), amp=0.3)
+    st_o, fo, ho = _oracle(c, 20, True)
+    st_g, fg, hg = _gpu(gpu_lib, c, 20, modulate=True)
+    _, fp, hp = _oracle(c, 20, True, perturb=True)
+    bar = max(1e-12, 8.0 * max(rel_l2(fp, fo), rel_l2(hp, ho)))
+    ef, eh = rel_l2(fg, fo), rel_l2(hg, ho)
+    print(f"[spacing] {c.name} rk{rk} fm{fm}: f {ef:.2e} h {eh:.2e} bar {bar:.2e}", flush=True)
+    assert st_o == 0 and st_g == 0
+    assert ef <= bar and eh <= bar
+
+
 def test_batch_mixing_spaced_and_uniform_members(gpu_lib):
     a = cases.stretched(cases.shear_wave(32, u0=0.12, steps=20), amp=0.25)
     b = cases.shear_wave(48, u0=0.1, steps=20)

@@ -840,9 +840,12 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
 // MOM (with MODE 0 of RK3): also the moments of each fluid cell's stored
 // dF, dH over the directions -> in.dmom.
 //
-// Directions alternate between two staging buffers: while the threads
-// compute direction I's faces from one, the loads of direction I+1 stream
-// into the other, so the load latency hides behind the face arithmetic.
+// FP32 storage: directions alternate between two staging buffers filled by
+// direct-to-LDS loads; while the threads compute direction I's faces from
+// one, the loads of direction I+1 stream into the other. FP64 storage: one
+// buffer, written from registers; outside the moment kernel the loads of
+// direction I+1 go out into registers once direction I's faces are done,
+// so their latency hides behind direction I's net fluxes.
 // ---------------------------------------------------------------------------
 
 // Per-thread state of one tile pass. The instance fields the direction loop
@@ -860,7 +863,6 @@ struct TileWork {
   const double* alpha;
   long long P;
   double c;
-  TC rdx, rdy;
 };
 
 // Direction I of a tile.
@@ -1031,8 +1033,6 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
   W.P = in.plane;
   W.src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
   W.dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
-  W.rdx = TC(in.rdx);
-  W.rdy = TC(in.rdy);
   // own cells of this thread: cell ci = t + q NT at (ci % TX, ci / TX)
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
