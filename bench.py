@@ -349,8 +349,9 @@ def main() -> int:
 
     if "serve" in legs and args.serve_seconds > 0:
         # --- served request stream ---------------------------------------
-        # the survey's stream (offered load ~21x one GPU's FP64 capacity),
-        # then the same request distribution at a rate one GPU sustains
+        # the served workload's own rate (50 requests/s of sides log-uniform
+        # in [64, 1024]: offered load ~17x one GPU's FP64 capacity), then
+        # the same request distribution at a rate one GPU sustains
         result["serve"] = serve_leg(args.serve_rate, args.serve_seconds, 7, args.serve_tick, args.serve_max_wall)
         log(f"serve: {result['serve']}")
         if args.serve_sustained_seconds > 0:

@@ -147,8 +147,9 @@ def test_flux_shape_table():
 
 
 def test_integration_binding_compiles(tmp_path):
-    """The maintainer-side binding shown in INTEGRATION.md (gpuStep.h)
-    compiles against include/uqlbm.h, given a class with the public members
+    """The maintainer-side binding published with the repository (the
+    `gpuStep.h` block of the integration notes) compiles against
+    include/uqlbm.h, given a class with the public members
     of the reference's cpuLBM that it touches (declared here for the check
     only)."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
