@@ -38,7 +38,7 @@ import make_golden  # noqa: E402
 def deck(name: str, n: int, steps: int):
     if name == "riemann":
         return cases.riemann(n, steps=steps)
-    return cases.flat_plate(512, 256, steps=steps)   # config 2: plate from x = 64
+    return cases.flat_plate(512, 256, steps=steps)   # the 512 x 256 Mach-2 plate deck, plate from x = 64
 
 
 def _oracle(args):
