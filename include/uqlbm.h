@@ -177,6 +177,8 @@ int lbm_set_spacing(lbm_solver* s, const double* dx, const double* dy);
  * momentum and energy are conserved on closed (periodic) domains. */
 int lbm_get_totals(lbm_solver* s, double* out);
 
+/* Grid, derived constants (dt, c, Tc, b, K; cpuLBM's setupVars values,
+ * CPP/cpuLBM.cpp:360-450), step count and device footprint of a solver. */
 int lbm_get_info(const lbm_solver* s, lbm_info* out);
 
 void lbm_destroy(lbm_solver* s);
@@ -223,9 +225,14 @@ typedef struct lbm_timing {
  */
 #define LBM_FLUX_SHAPES 5
 int lbm_set_flux_shape(int32_t precision, int32_t shape);
+/* Own-cell width and height of a flux tile shape. */
 int lbm_flux_shape_dims(int32_t shape, int32_t* w, int32_t* h);
 int lbm_batch_autotune(lbm_batch* b, int32_t steps, int32_t* shape_out, double* ms_per_step_out);
 
+/* Per-kernel timing of later step calls (process-wide; 0 turns it off).
+ * While on, steps launch kernel by kernel instead of replaying graphs.
+ * lbm_get_timing copies the record of the last step call. No counterpart
+ * in the reference. */
 int lbm_set_kernel_timing(int32_t enable);
 int lbm_get_timing(lbm_timing* out);
 
@@ -269,12 +276,15 @@ int lbm_server_tick(lbm_server* sv, int32_t tick_steps, int32_t* active_out,
 /* Completed requests since the last poll (at most max). Returns the count
  * written, or a negative status. */
 int lbm_server_poll(lbm_server* sv, lbm_request_result* out, int32_t max);
-/* Fetch the final macroscopic fields of a completed request that was
- * submitted with keep_result enabled (see lbm_server_keep_results).
- * Fields are [x][y]; any pointer may be NULL. */
+/* With enable != 0, requests that complete from now on keep their final
+ * macroscopic fields in host memory (until the server is destroyed). */
 int lbm_server_keep_results(lbm_server* sv, int32_t enable);
+/* Copy the kept final fields of completed request `tag`, [x][y]; any
+ * pointer may be NULL. LBM_ERR_ARG when nothing was kept for the tag. */
 int lbm_server_result_macro(lbm_server* sv, int64_t tag, double* rho,
                             double* ux, double* uy, double* p, double* T);
+/* Lattice updates done so far and device bytes held by resident solves;
+ * either pointer may be NULL. */
 int lbm_server_stats(lbm_server* sv, int64_t* lups_done, size_t* hbm_in_use);
 /* Current host clock in seconds (the server's time base). */
 double lbm_clock_s(void);
