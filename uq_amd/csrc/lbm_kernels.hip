@@ -323,14 +323,15 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
   flag_failure(in, step, m3.rho, sumfe, eq_check_tol<TC>(sumabs));
 }
 
-// Two waves per SIMD: the FP64 instantiation otherwise takes 262 registers
-// (one wave per SIMD); capping it at 256 costs a few spills and hides twice
-// the memory latency (4096^2 FP64: 5.9 -> 4.2 ms per launch).
+// Two waves per SIMD for FP64: the instantiation otherwise takes 262
+// registers (one wave per SIMD); capping it at 256 costs a few spills and
+// hides twice the memory latency (4096^2 FP64: 5.9 -> 4.2 ms per launch).
+// Four for FP32 (128 registers, a few spills, instead of 150 and three):
+// half the bytes per load need more waves in flight (mixed ensemble: 2.30
+// -> 2.10 ms per launch).
 template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k_s3(const InstDev* insts,
-                                                                                   const TileRef* tiles,
-                                                                                   int ntiles, int par,
-                                                                                   const int* stepp) {
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 4 ? 4 : 2))) void k_s3(
+    const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
