@@ -328,7 +328,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
 // hides twice the memory latency (4096^2 FP64: 5.9 -> 4.2 ms per launch).
 // Four for FP32 (128 registers, a few spills, instead of 150 and three):
 // half the bytes per load need more waves in flight (mixed ensemble: 2.30
-// -> 2.10 ms per launch).
+// -> 2.10 ms per launch; five waves spill 176 bytes per lane: 2.91 ms).
 template <typename TS, typename TC>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 4 ? 4 : 2))) void k_s3(
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
