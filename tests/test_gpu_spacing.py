@@ -114,6 +114,25 @@ def test_batch_mixing_spaced_and_uniform_members(gpu_lib):
             s.close()
 
 
+def test_spacing_set_after_init_on_a_batch_member(gpu_lib):
+    """Spacing set after initialisation, on a solver already in a batch but
+    before its first step, reaches the batched kernels (the batch reads its
+    members' instance records at every step call)."""
+    c = cases.shear_wave(32, u0=0.12, steps=20)
+    sp = cases.stretched(c, amp=0.25)
+    st, f_ref, h_ref = _gpu(gpu_lib, sp, 20)
+    s = gpu_lib.Solver(c, "fp64")
+    bt = gpu_lib.Batch([s])
+    try:
+        s.set_spacing(sp.meta["spacing"]["dx"], sp.meta["spacing"]["dy"])
+        assert st == 0 and bt.step(20) == 0
+        f, h = s.get_dist()
+        assert np.array_equal(f, f_ref) and np.array_equal(h, h_ref)
+    finally:
+        bt.close()
+        s.close()
+
+
 def test_mixed_precision_on_a_stretched_grid(gpu_lib):
     c = cases.stretched(cases.shear_wave(48, u0=0.15, steps=30), amp=0.3)
     st64, f64, _ = _gpu(gpu_lib, c, 30)
