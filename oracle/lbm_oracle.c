@@ -407,7 +407,16 @@ static double net_flux(const orc* o, const double* flux, int x, int y, int i) {
   const int xu = map_index(o, x, -sx, 0, &mir_y);
   const int yu = map_index(o, y, -sy, 1, &mir_x);
   mirror_dirs(i, mir_x, mir_y, &i_x, &i_y);
+#if defined(ORC_PLANT_SPACING) && ORC_PLANT_SPACING == 1
+  /* planted mistake, built only by the pin test: the upwind column's dx */
+  const double rx = o->dt / o->dxs[xu], ry = o->dt / o->dys[y];
+#elif defined(ORC_PLANT_SPACING) && ORC_PLANT_SPACING == 2
+  /* planted mistake, built only by the pin test: dx and dy indices swapped
+   * (valid on square grids only) */
+  const double rx = o->dt / o->dxs[y], ry = o->dt / o->dys[x];
+#else
   const double rx = o->dt / o->dxs[x], ry = o->dt / o->dys[y];
+#endif
   const double out = rx * flux[2 * DIST(o, x, y, i)] + ry * flux[2 * DIST(o, x, y, i) + 1];
   const double in = rx * flux[2 * DIST(o, xu, y, i_y)] + ry * flux[2 * DIST(o, x, yu, i_x) + 1];
   return out - in;

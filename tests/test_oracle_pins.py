@@ -8,6 +8,7 @@ order of accuracy of the time stepper. Each test is chosen so that a
 plausible slip (a dropped term, a sign, a transposed index) fails it.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -472,6 +473,41 @@ def test_spacing_conserves_area_weighted_totals(rk, fm):
     start.close()
     w = f_spaced.sum()
     assert abs(w - w0) / w > 1e-12
+
+
+def _planted_oracle(tmp_path, plant):
+    """The oracle built with a planted spacing mistake (ORC_PLANT_SPACING),
+    loaded through a private copy of the pyoracle module."""
+    import importlib.util
+    import subprocess
+    here = os.path.dirname(pyoracle.__file__)
+    so = tmp_path / f"liblbm_oracle_plant{plant}.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c11",
+                    "-D_POSIX_C_SOURCE=199309L", f"-DORC_PLANT_SPACING={plant}", "-shared", "-o", str(so),
+                    os.path.join(here, "lbm_oracle.c"), "-lm"], check=True)
+    spec = importlib.util.spec_from_file_location(f"pyoracle_plant{plant}", pyoracle.__file__)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.LIB_PATH = str(so)
+    return mod
+
+
+@pytest.mark.parametrize("plant", [1, 2], ids=["upwind_dx", "dx_dy_swapped"])
+def test_spacing_pin_catches_planted_mistakes(tmp_path, plant):
+    """The area-weighted conservation pin fails on an oracle built with a
+    wrong spacing index in the net flux."""
+    mod = _planted_oracle(tmp_path, plant)
+    c = cases.stretched(cases.shear_wave(32, u0=0.15, steps=40), amp=0.3)
+    dx = np.array(c.meta["spacing"]["dx"])
+    dy = np.array(c.meta["spacing"]["dy"])
+    o = mod.Oracle(c)
+    _modulate(o, c.nx, c.ny)
+    t0 = _weighted_totals(o, dx, dy)
+    o.step(40)
+    t1 = _weighted_totals(o, dx, dy)
+    o.close()
+    drift = np.abs(t1 - t0) / np.array([t0[0], t0[0], t0[0], t0[3]])
+    assert drift.max() > 1e-9, drift
 
 
 def test_spacing_time_step_and_refusals():
