@@ -7,7 +7,13 @@
 //     direction of consecutive cells issues one coalesced row segment;
 //   * cell type map: uint8 [ny][pitch] (0 fluid, 1 permanent, 2 solid);
 //   * wall closure factor alpha: double [ny][pitch] (read at solid cells).
-// Storage is FP64 or FP32 (mixed precision) per instance group.
+// Storage is FP64 or FP32 (mixed precision) per instance group. FP32
+// storage holds every distribution as its deviation from the instance's
+// reference equilibrium `ref` (one value per direction for f and for h),
+// and each state buffer carries four FP64 planes with the cell's conserved
+// moments (sum f, sum f e_x, sum f e_y, sum h): a reader restores
+// f = dev + ref in FP64 and corrects it so that its moments equal the
+// stored ones, so FP32 rounding never touches mass, momentum or energy.
 #pragma once
 
 #include <cstdint>
@@ -44,6 +50,13 @@ struct InstDev {
   int* fail;           // per-instance: first failing step (INT_MAX when none)
   double* dmom;        // 4 FP64 planes: moments of the stage-2 net fluxes of a
                        // fluid cell (sum dF, sum dF e_x c, sum dF e_y c, sum dH)
+  double* mom[2];      // FP32 storage: conserved moments of state[k] (4 FP64
+                       // planes each); null for FP64 storage
+  double ref[26];      // FP32 storage: reference equilibrium f0..f12, h0..h12
+                       // (zero for FP64 storage)
+  double refsum[4];    // FP32 storage: conserved sums of ref (mass, x and y
+                       // momentum, energy)
+  double i2c;          // 1/(2c): momentum correction per axis population pair
   // physics (reference setupVars, cpuLBM.cpp:360-450)
   double dt, dx, dy;
   double rdx, rdy;     // dt/dx, dt/dy

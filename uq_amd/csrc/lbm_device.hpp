@@ -46,31 +46,32 @@ __host__ __device__ constexpr int eq_sb(int i) { return (i == 7 || i == 8) ? -1 
 // ---------------------------------------------------------------------------
 // Equilibrium polynomials: the closed forms of the reference
 // (inlineFunctions.cpp:10-61), evaluated with products instead of pow() and
-// with the constant denominators applied as reciprocal products.
+// with the constant denominators applied as reciprocal products. Host and
+// device: the host forms the FP32-storage reference equilibrium with them.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T poly_f0(T rho, T u, T v, T t) {
+__host__ __device__ __forceinline__ T poly_f0(T rho, T u, T v, T t) {
   const T u2 = u * u, v2 = v * v;
   return (rho * T(0.25)) * (T(4) + T(10) * (t * t) + u2 * u2 - T(5) * v2 + v2 * v2 +
                          T(10) * t * (T(-1) + u2 + v2) + u2 * (T(-5) + T(4) * v2));
 }
 template <typename T>
-__device__ __forceinline__ T poly_f1(T rho, T a, T b, T t) {
+__host__ __device__ __forceinline__ T poly_f1(T rho, T a, T b, T t) {
   const T a2 = a * a, b2 = b * b;
   return (rho * T(1.0 / 6.0)) * (T(-6) * (t * t) - a * (T(1) + a) * (T(-4) + a2 + T(3) * b2) -
                          t * (T(-4) + T(6) * a + T(9) * a2 + T(3) * b2));
 }
 template <typename T>
-__device__ __forceinline__ T poly_f5(T rho, T a, T b, T t) {
+__host__ __device__ __forceinline__ T poly_f5(T rho, T a, T b, T t) {
   return (rho * T(0.25)) * ((t + a + a * a) * (t + b + b * b));
 }
 template <typename T>
-__device__ __forceinline__ T poly_f9(T rho, T a, T t) {
+__host__ __device__ __forceinline__ T poly_f9(T rho, T a, T t) {
   return (rho * T(1.0 / 24.0)) * (T(3) * (t * t) + (T(-1) + a) * a * (T(1) + a) * (T(2) + a) +
                           t * (T(-1) + T(6) * a * (T(1) + a)));
 }
 template <typename T>
-__device__ __forceinline__ T poly_h0(T K, T t, T u, T v) {
+__host__ __device__ __forceinline__ T poly_h0(T K, T t, T u, T v) {
   const T u2 = u * u, v2 = v * v, u4 = u2 * u2, v4 = v2 * v2, t2 = t * t, t3 = t2 * t;
   return (T(10) * (T(16) + T(3) * K) * t3 +
           T(3) * t *
@@ -81,7 +82,7 @@ __device__ __forceinline__ T poly_h0(T K, T t, T u, T v) {
          T(1.0 / 24.0);
 }
 template <typename T>
-__device__ __forceinline__ T poly_h1(T K, T t, T a, T b) {
+__host__ __device__ __forceinline__ T poly_h1(T K, T t, T a, T b) {
   const T a2 = a * a, a3 = a2 * a, b2 = b * b, t2 = t * t, t3 = t2 * t;
   return (T(-2) * (T(16) + T(3) * K) * t3 - a * (T(1) + a) * (a2 + b2) * (T(-4) + a2 + T(3) * b2) -
           t * (a * (T(-4) * (T(4) + K) - T(4) * (T(7) + K) * a + (T(14) + K) * a2 + (T(19) + K) * a3) +
@@ -91,7 +92,7 @@ __device__ __forceinline__ T poly_h1(T K, T t, T a, T b) {
          T(1.0 / 12.0);
 }
 template <typename T>
-__device__ __forceinline__ T poly_h5(T K, T t, T a, T b) {
+__host__ __device__ __forceinline__ T poly_h5(T K, T t, T a, T b) {
   const T a2 = a * a, b2 = b * b, t2 = t * t, t3 = t2 * t;
   return ((T(16) + T(3) * K) * t3 + T(3) * a * (T(1) + a) * b * (T(1) + b) * (a2 + b2) +
           T(3) * t2 * ((T(6) + K) * a + (T(9) + K) * a2 + b * (T(6) + K + (T(9) + K) * b)) +
@@ -101,7 +102,7 @@ __device__ __forceinline__ T poly_h5(T K, T t, T a, T b) {
          T(1.0 / 24.0);
 }
 template <typename T>
-__device__ __forceinline__ T poly_h9(T K, T t, T a, T b) {
+__host__ __device__ __forceinline__ T poly_h9(T K, T t, T a, T b) {
   const T a2 = a * a, b2 = b * b, t2 = t * t, t3 = t2 * t;
   return ((T(16) + T(3) * K) * t3 +
           t * a * (T(-8) + K * (T(-1) + a) * (T(1) + a) * (T(2) + a) + a * (T(-7) + T(2) * a * (T(11) + T(8) * a))) +
@@ -122,7 +123,7 @@ struct EqIn {
 
 // f and h equilibrium of direction I (compile-time).
 template <int I, typename T>
-__device__ __forceinline__ void eq_dir(const EqIn<T>& q, T& fe, T& he) {
+__host__ __device__ __forceinline__ void eq_dir(const EqIn<T>& q, T& fe, T& he) {
   constexpr int base = eq_base(I);
   const T a = T(eq_sa(I)) * (eq_swap(I) ? q.v : q.u);
   const T b = T(eq_sb(I)) * (eq_swap(I) ? q.u : q.v);

@@ -122,16 +122,7 @@ __device__ __forceinline__ double eq_check_tol(double sum_abs) {
 // The pointwise kernels read each input plane once: non-temporal loads keep
 // them from displacing the lines the stencil kernels reuse (k_s12 and k_s3
 // 2-2.5 % faster).
-template <typename TS>
-__device__ __forceinline__ void load26(const TS* src, long long P, long long o, double* f, double* h) {
-#pragma unroll
-  for (int i = 0; i < kQ; i++) {
-    f[i] = double(__builtin_nontemporal_load(&src[P * i + o]));
-    h[i] = double(__builtin_nontemporal_load(&src[P * (kQ + i) + o]));
-  }
-}
-
-__device__ __forceinline__ void moments26(const InstDev& in, const double* f, const double* h, Macro& m) {
+__device__ __forceinline__ void sums26(const InstDev& in, const double* f, const double* h, double* s) {
   double rho = 0, rux = 0, ruy = 0, sh = 0;
 #pragma unroll
   for (int i = 0; i < kQ; i++) {
@@ -140,7 +131,126 @@ __device__ __forceinline__ void moments26(const InstDev& in, const double* f, co
     ruy += f[i] * (lat_y(i) * in.c);
     sh += h[i];
   }
-  finish_macro(in, rho, rux, ruy, sh, m);
+  s[0] = rho;
+  s[1] = rux;
+  s[2] = ruy;
+  s[3] = sh;
+}
+
+__device__ __forceinline__ void moments26(const InstDev& in, const double* f, const double* h, Macro& m) {
+  double s[4];
+  sums26(in, f, h, s);
+  finish_macro(in, s[0], s[1], s[2], s[3], m);
+}
+
+// The state of one cell and its conserved sums s (mass, x and y momentum,
+// energy). FP64 storage: the stored values and their sums. FP32 storage:
+// the stored deviations plus the reference equilibrium, corrected so that
+// the sums equal the FP64 moments kept beside the state: the mass defect
+// goes to the rest population f0, each momentum defect to the axis pair of
+// its axis (+d/(2c) on f1 or f2, -d/(2c) on f3 or f4, which moves no mass),
+// the energy defect to h0. The corrections are of the size of the FP32
+// rounding; they keep that rounding out of the conserved quantities. The
+// stored values stay in their storage type until a direction uses them
+// (value<I>), which keeps the FP32 loads at half the registers.
+template <typename TS>
+struct CellState {
+  TS f[kQ], h[kQ];
+  double s[4];
+  double cf[5], ch;  // FP32 storage: corrections of f0..f4 and h0
+};
+
+template <typename TS>
+__device__ __forceinline__ void load_state(const InstDev& in, const TS* src, const double* mom, long long o,
+                                           CellState<TS>& c) {
+  const long long P = in.plane;
+#pragma unroll
+  for (int i = 0; i < kQ; i++) {
+    c.f[i] = __builtin_nontemporal_load(&src[P * i + o]);
+    c.h[i] = __builtin_nontemporal_load(&src[P * (kQ + i) + o]);
+  }
+  double t[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < kQ; i++) {
+    const double fv = double(c.f[i]);
+    t[0] += fv;
+    t[1] += fv * (lat_x(i) * in.c);
+    t[2] += fv * (lat_y(i) * in.c);
+    t[3] += double(c.h[i]);
+  }
+  if constexpr (sizeof(TS) == 4) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) t[k] += in.refsum[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c.s[k] = __builtin_nontemporal_load(&mom[P * k + o]);
+    const double dx = (c.s[1] - t[1]) * in.i2c, dy = (c.s[2] - t[2]) * in.i2c;
+    c.cf[0] = c.s[0] - t[0];
+    c.cf[1] = dx;
+    c.cf[3] = -dx;
+    c.cf[2] = dy;
+    c.cf[4] = -dy;
+    c.ch = c.s[3] - t[3];
+  } else {
+    (void)mom;
+#pragma unroll
+    for (int k = 0; k < 4; k++) c.s[k] = t[k];
+  }
+}
+
+// f_I and h_I of a loaded cell in FP64, in the stored form: FP32 storage
+// works on deviations from the reference throughout the pointwise stages
+// (every stage update is an affine combination of distribution values with
+// weights summing to one, plus differences, so it maps deviations to
+// deviations; only the equilibria need the reference taken off).
+template <int I, typename TS>
+__device__ __forceinline__ double fval(const CellState<TS>& c) {
+  if constexpr (sizeof(TS) == 4) {
+    double v = double(c.f[I]);
+    if constexpr (I < 5) v += c.cf[I];
+    return v;
+  } else {
+    return c.f[I];
+  }
+}
+template <int I, typename TS>
+__device__ __forceinline__ double hval(const CellState<TS>& c) {
+  if constexpr (sizeof(TS) == 4) {
+    double v = double(c.h[I]);
+    if constexpr (I == 0) v += c.ch;
+    return v;
+  } else {
+    return c.h[I];
+  }
+}
+
+// Equilibrium of direction I in the stored form (FP32 storage: minus the
+// reference); with SUM, the full f value joins the failure-check sums.
+template <int I, typename TS, bool SUM>
+__device__ __forceinline__ void eq_stored(const InstDev& in, const EqIn<double>& q, double& fe, double& he,
+                                          double& sumfe, double& sumabs) {
+  eq_dir<I>(q, fe, he);
+  if constexpr (SUM) {
+    sumfe += fe;
+    sumabs += fabs(fe);
+  }
+  if constexpr (sizeof(TS) == 4) {
+    fe -= in.ref[I];
+    he -= in.ref[kQ + I];
+  }
+}
+
+// Stored value of a full distribution value computed in FP64: the
+// deviation from the reference for FP32 storage.
+template <typename TS>
+__device__ __forceinline__ TS to_store(const InstDev& in, int k, double v) {
+  if constexpr (sizeof(TS) == 4) return TS(v - in.ref[k]);
+  else return TS(v);
+}
+
+// The state buffer k of an instance and its moment planes.
+template <typename TS>
+__device__ __forceinline__ const TS* state_buf(const InstDev& in, int k) {
+  return reinterpret_cast<const TS*>(in.state[k]);
 }
 
 // Stage-1/2 coefficients of one cell. Every divisor of the stage updates is
@@ -172,8 +282,7 @@ __device__ __forceinline__ Rec12<TC> make_rec12(const InstDev& in, const Macro& 
 // f2, h2 of direction I (reference stages 1-2, cpuLBM.cpp:1853, 1862, 1905,
 // 1920) and the stage-1 equilibria fe, he (feq2 = feq1).
 template <int I, typename TC>
-__device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC& fe, TC& he, TC& f2, TC& h2) {
-  eq_dir<I>(r.q, fe, he);
+__device__ __forceinline__ void s12(const Rec12<TC>& r, TC hdt, TC fv, TC hv, TC fe, TC he, TC& f2, TC& h2) {
   const TC f1 = (fv + r.af * fe) * r.iaf;
   const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
   const TC h1 = (hv - r.ahf * eu * (fe - f1) + r.ah * he) * r.iah;
@@ -201,22 +310,21 @@ __device__ __forceinline__ PointCell point_cell(const InstDev& in, const TileRef
 // k_s12: RK3 stages 1 and 2, pointwise -> f2.
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, TS* dst, long long o, int step,
-                                         Seq<Is...>) {
+__device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const double* mom, TS* dst, long long o,
+                                         int step, Seq<Is...>) {
   const long long P = in.plane;
-  double f[kQ], h[kQ];
-  load26(src, P, o, f, h);
+  CellState<TS> cs;
+  load_state(in, src, mom, o, cs);
   Macro m;
-  moments26(in, f, h, m);
+  finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
   const Rec12<TC> r = make_rec12<TC>(in, m);
   const TC hdt = TC(in.dt) / TC(2);
   double sumfe = 0, sumabs = 0;
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC fe, he, f2, h2;
-    s12<I>(r, hdt, TC(f[I]), TC(h[I]), fe, he, f2, h2);
-    sumfe += double(fe);
-    sumabs += fabs(double(fe));
+    eq_stored<I, TS, true>(in, r.q, fe, he, sumfe, sumabs);
+    s12<I>(r, hdt, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f2, h2);
     // streaming stores: f2 and, below, f3 and G are written once per step
     // and read back only by the next kernel, long after L2 has turned over
     __builtin_nontemporal_store(TS(f2), &dst[P * I + o]);
@@ -236,14 +344,15 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
     if (frozen_by_failure(in, step)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
-    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+    const int cur = in.parity0 ^ par;
+    const TS* src = state_buf<TS>(in, cur);
     TS* dst = reinterpret_cast<TS*>(in.f2);
     if (in.ctype[c.o] > 0) {
 #pragma unroll
       for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
       continue;
     }
-    s12_cell<TS, TC>(in, src, dst, c.o, step, MakeSeq<kQ>::type{});
+    s12_cell<TS, TC>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
@@ -259,29 +368,33 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
 // f - dF2/2 + dt/2 (feq2 - f2)/tf2 since af2 = dt/(2 tf2).)
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const TS* dfb, TS* f3b, TS* out,
-                                        long long o, int step, Seq<Is...>) {
+__device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb,
+                                        TS* f3b, TS* out, double* mom_out, long long o, int step, Seq<Is...>) {
   const long long P = in.plane;
-  double f[kQ], h[kQ];
-  load26(src, P, o, f, h);
+  CellState<TS> cs;
+  load_state(in, src, mom, o, cs);
+  const double* s = cs.s;
   // stage-1 moments, and the stage-3 moments: those of f - dF(f2)
   // (macroProp3, cpuLBM.cpp:973-1032), formed as moments of f minus the
   // moments of dF that the stage-2 flux kernel summed for this cell, so that
   // every direction can be finished in one pass
-  double rho = 0, rux = 0, ruy = 0, sh = 0;
-#pragma unroll
-  for (int i = 0; i < kQ; i++) {
-    rho += f[i];
-    rux += f[i] * (lat_x(i) * in.c);
-    ruy += f[i] * (lat_y(i) * in.c);
-    sh += h[i];
-  }
   const double drho = __builtin_nontemporal_load(&in.dmom[o]), drux = __builtin_nontemporal_load(&in.dmom[P + o]),
                druy = __builtin_nontemporal_load(&in.dmom[2 * P + o]),
                dsh = __builtin_nontemporal_load(&in.dmom[3 * P + o]);
   Macro m1, m3;
-  finish_macro(in, rho, rux, ruy, sh, m1);
-  finish_macro(in, rho - drho, rux - drux, ruy - druy, sh - dsh, m3);
+  finish_macro(in, s[0], s[1], s[2], s[3], m1);
+  finish_macro(in, s[0] - drho, s[1] - drux, s[2] - druy, s[3] - dsh, m3);
+  if constexpr (sizeof(TS) == 4) {
+    // conserved moments of G = the new state before its dF(f3)/2 term: the
+    // relaxation terms move none, so they are those of f - dF(f2)/2; the
+    // stage-3 flux kernel subtracts half those of dF(f3)
+    mom_out[o] = s[0] - 0.5 * drho;
+    mom_out[P + o] = s[1] - 0.5 * drux;
+    mom_out[2 * P + o] = s[2] - 0.5 * druy;
+    mom_out[3 * P + o] = s[3] - 0.5 * dsh;
+  } else {
+    (void)mom_out;
+  }
   const Rec12<TC> r = make_rec12<TC>(in, m1);
   const TC hdt = TC(in.dt) * TC(0.5);
   const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
@@ -296,8 +409,9 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
       dF = TC(__builtin_nontemporal_load(&dfb[P * (I - 1) + o]));
       dH = TC(__builtin_nontemporal_load(&dfb[P * (12 + I - 1) + o]));
     }
-    const TC fv = TC(f[I]), hv = TC(h[I]);
+    const TC fv = TC(fval<I>(cs)), hv = TC(hval<I>(cs));
     TC fe, he, f2, h2;
+    eq_stored<I, TS, false>(in, r.q, fe, he, sumfe, sumabs);
     s12<I>(r, hdt, fv, hv, fe, he, f2, h2);
     const TC D = fe - f2;
     const TC eu2 = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
@@ -305,9 +419,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     const TC Pf = (fv - dF) + r.af * D;
     const TC Qh = ((hv - dH) - hdt * A2) + r.ah * (he - h2);
     TC fe3, he3;
-    eq_dir<I>(q3, fe3, he3);
-    sumfe += double(fe3);
-    sumabs += fabs(double(fe3));
+    eq_stored<I, TS, true>(in, q3, fe3, he3, sumfe, sumabs);
     const TC f3 = (Pf + af3 * fe3) * iaf3;
     const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
     const TC A3 = (eu3 * ithf3) * (fe3 - f3);
@@ -318,19 +430,36 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     __builtin_nontemporal_store(TS(h3), &f3b[P * (kQ + I) + o]);
     __builtin_nontemporal_store(TS(gf), &out[P * I + o]);
     __builtin_nontemporal_store(TS(gh), &out[P * (kQ + I) + o]);
+    // one direction at a time: keeps the scheduler from interleaving the
+    // directions' temporaries (FP32 storage: 38 spilled registers otherwise)
+    if constexpr (sizeof(TS) == 4) __builtin_amdgcn_sched_barrier(0);
   };
   (one(std::integral_constant<int, Is>{}), ...);
   flag_failure(in, step, m3.rho, sumfe, eq_check_tol<TC>(sumabs));
 }
 
-// Two waves per SIMD for FP64: the instantiation otherwise takes 262
-// registers (one wave per SIMD); capping it at 256 costs a few spills and
-// hides twice the memory latency (4096^2 FP64: 5.9 -> 4.2 ms per launch).
-// Four for FP32 (128 registers, a few spills, instead of 150 and three):
-// half the bytes per load need more waves in flight (mixed ensemble: 2.30
-// -> 2.10 ms per launch; five waves spill 176 bytes per lane: 2.91 ms).
+// Frozen cell (permanent or solid) of a pointwise stage: the stored values
+// (and, for FP32 storage, the moment planes) carried over unchanged.
+template <typename TS>
+__device__ __forceinline__ void copy_frozen(const InstDev& in, const TS* src, TS* dst, int from, int to,
+                                            long long o) {
+#pragma unroll
+  for (int k = 0; k < kPlanes; k++) dst[in.plane * k + o] = src[in.plane * k + o];
+  if constexpr (sizeof(TS) == 4) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) in.mom[to][in.plane * k + o] = in.mom[from][in.plane * k + o];
+  } else {
+    (void)from;
+    (void)to;
+  }
+}
+
+// Two waves per SIMD (FP64 arithmetic for both storage types): the
+// instantiation otherwise takes 262 registers (one wave per SIMD); capping
+// it at 256 costs a few spills and hides twice the memory latency (4096^2
+// FP64: 5.9 -> 4.2 ms per launch).
 template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 4 ? 4 : 2))) void k_s3(
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(sizeof(TC) == 4 ? 4 : 2))) void k_s3(
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
@@ -339,20 +468,19 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) 
     if (frozen_by_failure(in, step)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
-    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
-    TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+    const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
+    const TS* src = state_buf<TS>(in, cur);
+    TS* out = reinterpret_cast<TS*>(in.state[nxt]);
     TS* f3b = reinterpret_cast<TS*>(in.f3);
     if (in.ctype[c.o] > 0) {
       // frozen cell: f3 = f2 = f, and the new state is f
 #pragma unroll
-      for (int k = 0; k < kPlanes; k++) {
-        const TS v = src[in.plane * k + c.o];
-        f3b[in.plane * k + c.o] = v;
-        out[in.plane * k + c.o] = v;
-      }
+      for (int k = 0; k < kPlanes; k++) f3b[in.plane * k + c.o] = src[in.plane * k + c.o];
+      copy_frozen(in, src, out, cur, nxt, c.o);
       continue;
     }
-    s3_cell<TS, TC>(in, src, reinterpret_cast<const TS*>(in.dflux), f3b, out, c.o, step, MakeSeq<kQ>::type{});
+    s3_cell<TS, TC>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), f3b, out, in.mom[nxt], c.o, step,
+                    MakeSeq<kQ>::type{});
   }
 }
 
@@ -388,29 +516,27 @@ __device__ __forceinline__ Rec1<TC> make_rec1(const InstDev& in, const Macro& m)
 
 // implicit stage of RK1 (cpuLBM.cpp:1650, 1660)
 template <int I, typename TC>
-__device__ __forceinline__ void s1(const Rec1<TC>& r, TC fv, TC hv, TC& fe, TC& he, TC& f1, TC& h1) {
-  eq_dir<I>(r.q, fe, he);
+__device__ __forceinline__ void s1(const Rec1<TC>& r, TC fv, TC hv, TC fe, TC he, TC& f1, TC& h1) {
   f1 = (fv + r.bf * fe) * r.ibf;
   const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
   h1 = (hv - r.dt * eu * ((fe - f1) * r.ithf) + r.bh * he) * r.ibh;
 }
 
 template <typename TS, typename TC, int... Is>
-__device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, TS* dst, long long o, int step,
-                                        Seq<Is...>) {
+__device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, const double* mom, TS* dst, long long o,
+                                        int step, Seq<Is...>) {
   const long long P = in.plane;
-  double f[kQ], h[kQ];
-  load26(src, P, o, f, h);
+  CellState<TS> cs;
+  load_state(in, src, mom, o, cs);
   Macro m;
-  moments26(in, f, h, m);
+  finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
   const Rec1<TC> r = make_rec1<TC>(in, m);
   double sumfe = 0, sumabs = 0;
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC fe, he, f1, h1;
-    s1<I>(r, TC(f[I]), TC(h[I]), fe, he, f1, h1);
-    sumfe += double(fe);
-    sumabs += fabs(double(fe));
+    eq_stored<I, TS, true>(in, r.q, fe, he, sumfe, sumabs);
+    s1<I>(r, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f1, h1);
     dst[P * I + o] = TS(f1);
     dst[P * (kQ + I) + o] = TS(h1);
   };
@@ -428,29 +554,31 @@ __global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef*
     if (frozen_by_failure(in, step)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
-    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
+    const int cur = in.parity0 ^ par;
+    const TS* src = state_buf<TS>(in, cur);
     TS* dst = reinterpret_cast<TS*>(in.f2);
     if (in.ctype[c.o] > 0) {
 #pragma unroll
       for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
       continue;
     }
-    s1_cell<TS, TC>(in, src, dst, c.o, step, MakeSeq<kQ>::type{});
+    s1_cell<TS, TC>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
 // Final update of the one-flux-stage steppers, METHOD 0 (RK1, cpuLBM.cpp:
 // 1696, 1707), 2 (EULER, :2155, :2165), 3 (EULER_EQ, :2245, :2255).
 template <int METHOD, typename TS, typename TC, int... Is>
-__device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const TS* dfb, TS* out, long long o,
-                                         int step, Seq<Is...>) {
+__device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb, TS* out,
+                                         double* mom_out, long long o, int step, Seq<Is...>) {
   const long long P = in.plane;
-  double f[kQ], h[kQ];
-  load26(src, P, o, f, h);
+  CellState<TS> cs;
+  load_state(in, src, mom, o, cs);
   Macro m;
-  moments26(in, f, h, m);
+  finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
   const Rec1<TC> r = make_rec1<TC>(in, m);
   double sumfe = 0, sumabs = 0;
+  double t[4] = {0, 0, 0, 0};  // conserved sums of the new state (FP32 storage)
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC dF = 0, dH = 0;
@@ -458,19 +586,18 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
       dF = TC(__builtin_nontemporal_load(&dfb[P * (I - 1) + o]));
       dH = TC(__builtin_nontemporal_load(&dfb[P * (12 + I - 1) + o]));
     }
-    const TC fv = TC(f[I]), hv = TC(h[I]);
+    const TC fv = TC(fval<I>(cs)), hv = TC(hval<I>(cs));
     const TC eu = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
     TC nf, nh;
     if constexpr (METHOD == 0) {
       TC fe, he, f1, h1;
+      eq_stored<I, TS, false>(in, r.q, fe, he, sumfe, sumabs);
       s1<I>(r, fv, hv, fe, he, f1, h1);
       nf = fv - dF + r.bf * (fe - f1);
       nh = hv - dH + r.bh * (he - h1) - ((r.dt * eu) * r.ithf) * (fe - f1);
     } else {
       TC fe, he;
-      eq_dir<I>(r.q, fe, he);
-      sumfe += double(fe);
-      sumabs += fabs(double(fe));
+      eq_stored<I, TS, true>(in, r.q, fe, he, sumfe, sumabs);
       if constexpr (METHOD == 2) {
         nf = fv - dF + r.bf * (fe - fv);
         nh = hv - dH + r.bh * (he - hv) - ((r.dt * eu) * r.ithf) * (fe - fv);
@@ -479,10 +606,22 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
         nh = he - dH;
       }
     }
+    if constexpr (sizeof(TS) == 4) {  // nf, nh are deviations: the reference's sums are added below
+      t[0] += double(nf);
+      t[1] += double(nf) * (lat_x(I) * in.c);
+      t[2] += double(nf) * (lat_y(I) * in.c);
+      t[3] += double(nh);
+    }
     out[P * I + o] = TS(nf);
     out[P * (kQ + I) + o] = TS(nh);
   };
   (one(std::integral_constant<int, Is>{}), ...);
+  if constexpr (sizeof(TS) == 4) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) mom_out[P * k + o] = t[k] + in.refsum[k];
+  } else {
+    (void)mom_out;
+  }
   if (METHOD != 0) flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
 }
 
@@ -496,15 +635,15 @@ __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef
     if (frozen_by_failure(in, step)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
-    const TS* src = reinterpret_cast<const TS*>(in.state[in.parity0 ^ par]);
-    TS* out = reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]);
+    const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
+    const TS* src = state_buf<TS>(in, cur);
+    TS* out = reinterpret_cast<TS*>(in.state[nxt]);
     if (in.ctype[c.o] > 0) {
-#pragma unroll
-      for (int k = 0; k < kPlanes; k++) out[in.plane * k + c.o] = src[in.plane * k + c.o];
+      copy_frozen(in, src, out, cur, nxt, c.o);
       continue;
     }
-    fin_cell<METHOD, TS, TC>(in, src, reinterpret_cast<const TS*>(in.dflux), out, c.o, step,
-                             MakeSeq<kQ>::type{});
+    fin_cell<METHOD, TS, TC>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), out, in.mom[nxt], c.o,
+                             step, MakeSeq<kQ>::type{});
   }
 }
 
@@ -548,6 +687,7 @@ struct Region {
   unsigned char meta[Sh::RY][Sh::RX];
   double rdx[Sh::TX], rdy[Sh::TY];  // dt/dx of the tile's columns, dt/dy of its rows (non-uniform grids)
   float rdxf[Sh::TX], rdyf[Sh::TY];  // the same in FP32, for the FP32 stencil arithmetic
+  float ref[kPlanes];               // FP32 storage: the instance's reference equilibrium
 };
 
 template <typename Sh>
@@ -575,6 +715,7 @@ __device__ void setup_region(Region<Sh>& R, const InstDev& in, int x0, int y0) {
     m |= cy ? kMetaClampY : 0u;
     R.meta[ly + kH][lx + kH] = (unsigned char)m;
   }
+  for (int k = threadIdx.x; k < kPlanes; k += Sh::NT) R.ref[k] = float(in.ref[k]);
   for (int k = threadIdx.x; k < Sh::TX + Sh::TY; k += Sh::NT) {  // dt/dx, dt/dy of the tile's columns and rows
     if (k < Sh::TX) {
       const double v = !in.rdxs ? in.rdx : (x0 + k < in.nx ? in.rdxs[x0 + k] : 0.0);
@@ -758,7 +899,8 @@ __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) 
 // staged, on a path of its own (no extrapolation state to carry).
 template <int AX, typename Sh, typename TC>
 __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, const double* alpha, int method,
-                                          bool solids, int lx, int ly, int sg, TC v, TC& vf, TC& vh) {
+                                          bool solids, int lx, int ly, int sg, TC v, TC rf, TC rh, TC& vf,
+                                          TC& vh) {
   const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
   if (!solids) {
     if (method == 1) {
@@ -778,10 +920,21 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
   }
   const unsigned mo = R.meta[ly + kH][lx + kH];
   if ((mo & kMetaType) == 2) {
+    // wall closure alpha |e| f_wall; staged deviations dev = f - ref give
+    // the deviation of that face, |e| (alpha dev + (alpha - 1) ref), since
+    // the |e| ref of every face cancels in the net flux (rf = rh = 0 for
+    // FP64 storage)
     const TC a = TC(alpha[R.off[ly + kH][lx + kH]]);
     const FH<TC> w = V[(ly + kH) * Sh::RX + lx + kH];
-    vf = a * v * w.f;
-    vh = a * v * w.h;
+    if constexpr (sizeof(TC) == 4) {
+      vf = v * (a * w.f + (a - TC(1)) * rf);
+      vh = v * (a * w.h + (a - TC(1)) * rh);
+    } else {
+      (void)rf;
+      (void)rh;
+      vf = a * v * w.f;
+      vh = a * v * w.h;
+    }
     return;
   }
   if (method == 1) {
@@ -886,6 +1039,9 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   const double* alpha = DB ? W.alpha : in.alpha;
   const int ex = c_lat_x[I], ey = c_lat_y[I];
   const int sx = ex > 0 ? 1 : (ex < 0 ? -1 : 0), sy = ey > 0 ? 1 : (ey < 0 ? -1 : 0);
+  // reference equilibrium of direction I for the wall faces of FP32 storage
+  // (a mirror image of I has the same reference: it depends on |e| only)
+  const TC rf = sizeof(TS) == 4 ? TC(R.ref[I]) : TC(0), rh = sizeof(TS) == 4 ? TC(R.ref[kQ + I]) : TC(0);
   if constexpr (!DB) {  // cur was last read before the previous barrier
     if constexpr (MOM) {
       SlotVals<Sh, TS> V;
@@ -903,14 +1059,14 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
     const int row = t / (Sh::TX + 1), j = t % (Sh::TX + 1);
     const TC v = TC(double(ex > 0 ? ex : -ex) * c);
     TC vf, vh;
-    face_flux<0>(cur.v, R, alpha, method, W.solids, j - (sx > 0 ? 1 : 0), row, sx, v, vf, vh);
+    face_flux<0>(cur.v, R, alpha, method, W.solids, j - (sx > 0 ? 1 : 0), row, sx, v, rf, rh, vf, vh);
     S.fx[row][j] = FH<TC>{vf, vh};
   }
   if (ey && t < Sh::NYF) {
     const int col = t % Sh::TX, j = t / Sh::TX;
     const TC v = TC(double(ey > 0 ? ey : -ey) * c);
     TC vf, vh;
-    face_flux<1>(cur.v, R, alpha, method, W.solids, col, j - (sy > 0 ? 1 : 0), sy, v, vf, vh);
+    face_flux<1>(cur.v, R, alpha, method, W.solids, col, j - (sy > 0 ? 1 : 0), sy, v, rf, rh, vf, vh);
     S.fy[j][col] = FH<TC>{vf, vh};
   }
   if constexpr (DB) __builtin_amdgcn_s_waitcnt(kWaitVmem);  // this wave's loads into nxt landed
@@ -1004,6 +1160,13 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
     } else {
       W.dst[P * I + o] = TS(TC(oldf[q]) - TC(0.5) * dF);
       W.dst[P * (kQ + I) + o] = TS(TC(oldh[q]) - TC(0.5) * dH);
+      if (MOM) {  // conserved moments of dF(f3), for the FP64 moment planes
+        const double dFs = double(dF);
+        S.mom[0][ci] += dFs;
+        S.mom[1][ci] += dFs * (ex * c);
+        S.mom[2][ci] += dFs * (ey * c);
+        S.mom[3][ci] += double(dH);
+      }
     }
   }
 }
@@ -1061,15 +1224,20 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
     for (int I = 1; I < kQ; I++) flux_dir<TS, TC, MODE, MOM, false>(S, R, in, W, L, PV, VA, VA, I, I + 1 < kQ);
   }
   if (MOM) {
+    // MODE 0: the moments of dF(f2) for stage 3. MODE 1 (FP32 storage): the
+    // new state's conserved moments, those of G (written by stage 3) minus
+    // half those of dF(f3).
+    double* mo = MODE == 1 ? in.mom[1 ^ in.parity0 ^ par] : in.dmom;
 #pragma unroll
     for (int q = 0; q < Sh::Cells; q++) {
       if (!W.fluid[q]) continue;
       const int ci = t + q * Sh::NT;
       const long long o = W.oo[q];
-      in.dmom[o] = S.mom[0][ci];
-      in.dmom[W.P + o] = S.mom[1][ci];
-      in.dmom[2 * W.P + o] = S.mom[2][ci];
-      in.dmom[3 * W.P + o] = S.mom[3][ci];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (MODE == 1) mo[k * W.P + o] -= 0.5 * S.mom[k][ci];
+        else mo[k * W.P + o] = S.mom[k][ci];
+      }
     }
   }
 }
@@ -1098,7 +1266,7 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
 // that reads of one do not wait for the loads into the other; the second
 // is dropped where it is unused.)
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
-__global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(MOM ? 2 : 8))) void k_flux(
+__global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(MOM && MODE == 0 ? 2 : 8))) void k_flux(
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, TC, MOM> S;
@@ -1150,6 +1318,13 @@ __device__ __forceinline__ int ref_index(const InstDev& in, int i, int pm, int d
   return r;
 }
 
+// Value of plane k at offset o in FP64 (FP32 storage: deviation + reference).
+template <typename TS>
+__device__ __forceinline__ double full_val(const InstDev& in, const TS* src, int k, long long o) {
+  if constexpr (sizeof(TS) == 4) return double(src[in.plane * k + o]) + in.ref[k];
+  else return double(src[in.plane * k + o]);
+}
+
 template <typename TS, typename TC>
 __device__ TC ref_face_flux_f(const InstDev& in, const TS* src, int x, int y, int i, int axis) {
   const int e = axis == 0 ? c_lat_x[i] : c_lat_y[i];
@@ -1165,7 +1340,7 @@ __device__ TC ref_face_flux_f(const InstDev& in, const TS* src, int x, int y, in
     const int p = ref_index(in, start, s * j, axis, mir);
     const int d = mir ? (axis == 0 ? c_mir_ew[i] : c_mir_ns[i]) : i;
     const long long o = axis == 0 ? (long long)y * in.pitch + p : (long long)p * in.pitch + x;
-    V[j] = TC(src[in.plane * d + o]);
+    V[j] = TC(full_val(in, src, d, o));
     sld[j] = in.ctype[o] == 2;
     any |= sld[j];
   }
@@ -1180,7 +1355,7 @@ __device__ TC ref_face_flux_f(const InstDev& in, const TS* src, int x, int y, in
     const int m2 = ref_index(in, pos, s * 2 * diff, axis, mir);
     const long long o1 = axis == 0 ? (long long)y * in.pitch + m1 : (long long)m1 * in.pitch + x;
     const long long o2 = axis == 0 ? (long long)y * in.pitch + m2 : (long long)m2 * in.pitch + x;
-    V[jj] = TC(2) * TC(src[in.plane * i + o1]) - TC(src[in.plane * i + o2]);
+    V[jj] = TC(2) * TC(full_val(in, src, i, o1)) - TC(full_val(in, src, i, o2));
     const int nx = jj - diff;
     if (nx >= 0 && nx < L) V[nx] = V[jj];
   }
@@ -1215,7 +1390,7 @@ __global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, 
       const double F = double(ref_face_flux_f<TS, TC>(in, src, xn, yn, i, axis));
       in_f += F / dd;
       const double vv = fabs(double(axis == 0 ? c_lat_x[io] : c_lat_y[io]) * in.c);
-      const double fw = double(src[in.plane * io + ow]);
+      const double fw = full_val(in, src, io, ow);
       out_f += fw * vv / dd;
     }
   }
@@ -1234,14 +1409,23 @@ __device__ __forceinline__ void init_cell(const InstDev& in, TS* dst, long long 
   m.uy = pr[2];
   m.T = pr[3];
   const EqIn<double> q = eq_inputs<double>(in, m);
+  double t[4] = {0, 0, 0, 0};
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     double fe, he;
     eq_dir<I>(q, fe, he);
-    dst[in.plane * I + o] = TS(fe);
-    dst[in.plane * (kQ + I) + o] = TS(he);
+    t[0] += fe;
+    t[1] += fe * (lat_x(I) * in.c);
+    t[2] += fe * (lat_y(I) * in.c);
+    t[3] += he;
+    dst[in.plane * I + o] = to_store<TS>(in, I, fe);
+    dst[in.plane * (kQ + I) + o] = to_store<TS>(in, kQ + I, he);
   };
   (one(std::integral_constant<int, Is>{}), ...);
+  if constexpr (sizeof(TS) == 4) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) in.mom[in.parity0][in.plane * k + o] = t[k];
+  }
 }
 
 // props: rows of {rho, ux, uy, T}; ids: [ny][pitch]
@@ -1266,10 +1450,10 @@ __global__ void k_macro(const InstDev* insts, int inst, int par, double* outp) {
   for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(o % in.pitch);
     if (x >= in.nx) continue;
-    double f[kQ], h[kQ];
-    load26(src, in.plane, o, f, h);
+    CellState<TS> cs;
+    load_state(in, src, in.mom[in.parity0 ^ par], o, cs);
     Macro m;
-    moments26(in, f, h, m);
+    finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
     outp[o] = m.rho;
     outp[n + o] = m.ux;
     outp[2 * n + o] = m.uy;
@@ -1315,6 +1499,12 @@ __global__ __launch_bounds__(256) void k_totals(const InstDev* insts, int inst, 
   double acc[4] = {0, 0, 0, 0};
   for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
     if ((int)(o % in.pitch) >= in.nx || in.ctype[o] != 0) continue;
+    if constexpr (sizeof(TS) == 4) {  // the FP64 moment planes of the state
+      const double* mo = in.mom[in.parity0 ^ par];
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[k] += mo[in.plane * k + o];
+      continue;
+    }
 #pragma unroll
     for (int i = 0; i < kQ; i++) {
       const double f = double(src[in.plane * i + o]);
@@ -1382,20 +1572,30 @@ static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 
 // Persistent grid sizes of every step kernel of one storage/compute type and
 // flux tile shape, computed once (before any stream capture).
+// Compute type of the pointwise stage kernels and the wall closure: FP64
+// for both storage types (the equilibria and stage updates of FP32-stored
+// deviations are formed in FP64 and rounded once, on the way out).
+using TP = double;
+
+// The stage-3 flux kernel also sums the moments of dF(f3) for the FP64
+// moment planes of FP32 storage.
+template <typename TS>
+constexpr bool kMom3 = sizeof(TS) == 4;
+
 template <typename TS, typename TC, typename Sh>
 struct StepGrids {
   int s12, flux2, s3, flux3, s1, flux0, fluxe, fin0, fin2, fin3;
   static const StepGrids& get() {
-    static const StepGrids g = {grid_for((const void*)k_s12<TS, TC>, kNT),
+    static const StepGrids g = {grid_for((const void*)k_s12<TS, TP>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s3<TS, TC>, kNT),
-                                grid_for((const void*)FluxKernel<TS, TC, 1, false, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s1<TS, TC>, kNT),
+                                grid_for((const void*)k_s3<TS, TP>, kNT),
+                                grid_for((const void*)FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn, Sh::NT),
+                                grid_for((const void*)k_s1<TS, TP>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT),
                                 grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_fin<TS, TC, 0>, kNT),
-                                grid_for((const void*)k_fin<TS, TC, 2>, kNT),
-                                grid_for((const void*)k_fin<TS, TC, 3>, kNT)};
+                                grid_for((const void*)k_fin<TS, TP, 0>, kNT),
+                                grid_for((const void*)k_fin<TS, TP, 2>, kNT),
+                                grid_for((const void*)k_fin<TS, TP, 3>, kNT)};
     return g;
   }
 };
@@ -1422,26 +1622,26 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
-  k_s12<TS, TC><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_s12<TS, TP><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(0, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, par, step);
+    k_alpha<TS, TP><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
   FluxKernel<TS, TC, 0, true, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_s3<TS, TC><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_s3<TS, TP><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, par, step);
+    k_alpha<TS, TP><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(4, s);
-  FluxKernel<TS, TC, 1, false, Sh>::fn<<<blocks_for(n, g3), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn<<<blocks_for(n, g3), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(4, s);
 }
 
@@ -1457,19 +1657,19 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
   const int wb = (L.nwalls + 255) / 256;
   if constexpr (METHOD == 0) {
     if (tm) tm->begin(0, s);
-    k_s1<TS, TC><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+    k_s1<TS, TP><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
     if (tm) tm->end(0, s);
   }
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TC><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, METHOD == 0 ? 0 : 2, par, step);
+    k_alpha<TS, TP><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, METHOD == 0 ? 0 : 2, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
   FluxKernel<TS, TC, FM, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_fin<TS, TC, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_fin<TS, TP, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
 }
 

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/uqlbm.h"
@@ -294,12 +295,51 @@ size_t uqlbm::solver_footprint(int nx, int ny, int precision) {
   const size_t pitch = (size_t)((nx + 63) / 64 * 64);
   const size_t plane = pitch * ny;
   const size_t es = elem_size(precision);
-  return plane * (kPlanes * 4 + dflux_planes()) * es + plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double));
+  const size_t mom = precision == LBM_PREC_FP64 ? 0 : 2 * 4 * sizeof(double);  // moment planes of FP32 storage
+  return plane * (kPlanes * 4 + dflux_planes()) * es +
+         plane * (1 + sizeof(double) + sizeof(int) + 5 * sizeof(double) + mom);
+}
+
+// Reference equilibrium of FP32 storage: the equilibrium at rest at the mean
+// density and temperature of the deck's fluid cells. Stored values are the
+// distributions' deviations from it; being at rest it depends on |e| only,
+// so a direction and its mirror images share it.
+template <int... Is>
+static void eq_all(const EqIn<double>& q, double* f, double* h, std::integer_sequence<int, Is...>) {
+  ((eq_dir<Is>(q, f[Is], h[Is])), ...);
+}
+
+static void set_reference(Solver& s) {
+  double rho = 0, T = 0;
+  long long n = 0;
+  for (int pass = 0; pass < 2 && n == 0; pass++)
+    for (size_t k = 0; k < s.ids_host.size(); k++) {
+      const PropRow& r = s.props[s.ids_host[k]];
+      if (pass == 0 && r.type != 0) continue;
+      rho += r.rho;
+      T += r.T;
+      n++;
+    }
+  rho /= (double)n;
+  T /= (double)n;
+  EqIn<double> q;
+  q.rho = rho;
+  q.u = 0;
+  q.v = 0;
+  q.t = T / s.Tc;
+  q.K = s.K;
+  q.scale = rho * s.p.R * s.Tc;
+  double f[kQ], h[kQ];
+  eq_all(q, f, h, std::make_integer_sequence<int, kQ>{});
+  for (int i = 0; i < kQ; i++) {
+    s.ref[i] = f[i];
+    s.ref[kQ + i] = h[i];
+  }
 }
 
 Solver::~Solver() {
   if (graphs) graphs->clear();
-  for (void* p : {state[0], state[1], f2, f3, (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
+  for (void* p : {state[0], state[1], f2, f3, (void*)mom[0], (void*)mom[1], (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
                   (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
                   (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev})
     if (p) (void)hipFree(p);
@@ -327,6 +367,18 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.alpha = s.alpha;
   d.fail = s.fail;
   d.dmom = s.macro_dev;  // planes 0-3 of the output buffer, free during a step
+  d.mom[0] = s.mom[0];
+  d.mom[1] = s.mom[1];
+  for (int k = 0; k < kPlanes; k++) d.ref[k] = s.precision == LBM_PREC_FP64 ? 0.0 : s.ref[k];
+  for (int k = 0; k < 4; k++) d.refsum[k] = 0.0;
+  if (s.precision != LBM_PREC_FP64)
+    for (int i = 0; i < kQ; i++) {
+      d.refsum[0] += s.ref[i];
+      d.refsum[1] += s.ref[i] * (lat_x(i) * s.c);
+      d.refsum[2] += s.ref[i] * (lat_y(i) * s.c);
+      d.refsum[3] += s.ref[kQ + i];
+    }
+  d.i2c = 0.5 / s.c;
   d.dt = s.dt;
   d.dx = s.p.dx;
   d.dy = s.p.dy;
@@ -490,6 +542,13 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   for (void** b : {&s->state[0], &s->state[1], &s->f2, &s->f3}) HIP_TRY(hipMalloc(b, plane * kPlanes * es));
   HIP_TRY(hipMalloc(&s->dflux, plane * dflux_planes() * es));
+  if (precision != LBM_PREC_FP64) {
+    for (double** m : {&s->mom[0], &s->mom[1]}) {
+      HIP_TRY(hipMalloc((void**)m, plane * 4 * sizeof(double)));
+      HIP_TRY(hipMemset(*m, 0, plane * 4 * sizeof(double)));
+    }
+    set_reference(*s);
+  }
   HIP_TRY(hipMalloc((void**)&s->ctype, plane));
   HIP_TRY(hipMalloc((void**)&s->alpha, plane * sizeof(double)));
   HIP_TRY(hipMalloc((void**)&s->fail, sizeof(int)));
@@ -789,36 +848,103 @@ extern "C" int lbm_get_totals(lbm_solver* h, double* out) {
   return LBM_OK;
 }
 
+// Host copies of the state: FP64 storage as stored; FP32 storage restores
+// deviation + reference and applies the moment correction of the kernels'
+// state loader (lbm_kernels.hip, load_state) with the FP64 moment planes.
+template <typename TS>
+static int read_state(Solver& s, std::vector<TS>& buf, std::vector<double>& mom) {
+  const size_t plane = (size_t)s.pitch * s.ny;
+  buf.resize(plane * kPlanes);
+  HIP_TRY(hipMemcpyAsync(buf.data(), s.state[s.cur], buf.size() * sizeof(TS), hipMemcpyDeviceToHost, s.stream));
+  if (s.precision != LBM_PREC_FP64) {
+    mom.resize(plane * 4);
+    HIP_TRY(hipMemcpyAsync(mom.data(), s.mom[s.cur], mom.size() * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  }
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return LBM_OK;
+}
+
+template <typename TS>
+static void cell_values(const Solver& s, const std::vector<TS>& buf, const std::vector<double>& mom, size_t o,
+                        double* f, double* h) {
+  const size_t plane = (size_t)s.pitch * s.ny;
+  for (int i = 0; i < kQ; i++) {
+    f[i] = double(buf[plane * i + o]);
+    h[i] = double(buf[plane * (kQ + i) + o]);
+  }
+  if (s.precision == LBM_PREC_FP64) return;
+  double t[4] = {0, 0, 0, 0};
+  for (int i = 0; i < kQ; i++) {
+    f[i] += s.ref[i];
+    h[i] += s.ref[kQ + i];
+    t[0] += f[i];
+    t[1] += f[i] * (lat_x(i) * s.c);
+    t[2] += f[i] * (lat_y(i) * s.c);
+    t[3] += h[i];
+  }
+  const double i2c = 0.5 / s.c;
+  const double dx = (mom[plane + o] - t[1]) * i2c, dy = (mom[2 * plane + o] - t[2]) * i2c;
+  f[0] += mom[o] - t[0];
+  f[1] += dx;
+  f[3] -= dx;
+  f[2] += dy;
+  f[4] -= dy;
+  h[0] += mom[3 * plane + o] - t[3];
+}
+
 template <typename TS>
 static int get_dist_t(Solver& s, double* f, double* hh) {
-  const size_t plane = (size_t)s.pitch * s.ny;
-  std::vector<TS> buf(plane * kPlanes);
-  HIP_TRY(hipMemcpyAsync(buf.data(), s.state[s.cur], buf.size() * sizeof(TS), hipMemcpyDeviceToHost, s.stream));
-  HIP_TRY(hipStreamSynchronize(s.stream));
+  std::vector<TS> buf;
+  std::vector<double> mom;
+  int st = read_state(s, buf, mom);
+  if (st != LBM_OK) return st;
   for (int x = 0; x < s.nx; x++)
-    for (int y = 0; y < s.ny; y++)
+    for (int y = 0; y < s.ny; y++) {
+      double fv[kQ], hv[kQ];
+      cell_values(s, buf, mom, (size_t)y * s.pitch + x, fv, hv);
       for (int i = 0; i < kQ; i++) {
-        const size_t o = (size_t)y * s.pitch + x, k = ((size_t)x * s.ny + y) * kQ + i;
-        if (f) f[k] = double(buf[plane * i + o]);
-        if (hh) hh[k] = double(buf[plane * (kQ + i) + o]);
+        const size_t k = ((size_t)x * s.ny + y) * kQ + i;
+        if (f) f[k] = fv[i];
+        if (hh) hh[k] = hv[i];
       }
+    }
   return LBM_OK;
 }
 
 template <typename TS>
 static int set_dist_t(Solver& s, const double* f, const double* hh) {
+  std::vector<TS> buf;
+  std::vector<double> mom;
+  int st = read_state(s, buf, mom);
+  if (st != LBM_OK) return st;
   const size_t plane = (size_t)s.pitch * s.ny;
-  std::vector<TS> buf(plane * kPlanes);
-  HIP_TRY(hipMemcpyAsync(buf.data(), s.state[s.cur], buf.size() * sizeof(TS), hipMemcpyDeviceToHost, s.stream));
-  HIP_TRY(hipStreamSynchronize(s.stream));
   for (int x = 0; x < s.nx; x++)
-    for (int y = 0; y < s.ny; y++)
+    for (int y = 0; y < s.ny; y++) {
+      const size_t o = (size_t)y * s.pitch + x;
+      double fv[kQ], hv[kQ];
+      cell_values(s, buf, mom, o, fv, hv);  // keeps the cell's values where f or h is not given
       for (int i = 0; i < kQ; i++) {
-        const size_t o = (size_t)y * s.pitch + x, k = ((size_t)x * s.ny + y) * kQ + i;
-        if (f) buf[plane * i + o] = TS(f[k]);
-        if (hh) buf[plane * (kQ + i) + o] = TS(hh[k]);
+        const size_t k = ((size_t)x * s.ny + y) * kQ + i;
+        if (f) fv[i] = f[k];
+        if (hh) hv[i] = hh[k];
       }
+      double t[4] = {0, 0, 0, 0};
+      for (int i = 0; i < kQ; i++) {
+        const double r = s.precision == LBM_PREC_FP64 ? 0.0 : s.ref[i];
+        const double rh = s.precision == LBM_PREC_FP64 ? 0.0 : s.ref[kQ + i];
+        buf[plane * i + o] = TS(fv[i] - r);
+        buf[plane * (kQ + i) + o] = TS(hv[i] - rh);
+        t[0] += fv[i];
+        t[1] += fv[i] * (lat_x(i) * s.c);
+        t[2] += fv[i] * (lat_y(i) * s.c);
+        t[3] += hv[i];
+      }
+      if (s.precision != LBM_PREC_FP64)
+        for (int k = 0; k < 4; k++) mom[k * plane + o] = t[k];
+    }
   HIP_TRY(hipMemcpyAsync(s.state[s.cur], buf.data(), buf.size() * sizeof(TS), hipMemcpyHostToDevice, s.stream));
+  if (s.precision != LBM_PREC_FP64)
+    HIP_TRY(hipMemcpyAsync(s.mom[s.cur], mom.data(), mom.size() * sizeof(double), hipMemcpyHostToDevice, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
   return LBM_OK;
 }
@@ -1063,11 +1189,14 @@ extern "C" int lbm_batch_autotune(lbm_batch* hb, int32_t steps, int32_t* shape_o
   for (size_t k = 0; k < b.members.size(); k++) {
     Solver& s = *b.members[k];
     const size_t bytes = (size_t)s.pitch * s.ny * kPlanes * es;
-    if (hipMalloc(&saved[k].buf, bytes) != hipSuccess) {
+    const size_t mbytes = s.mom[0] ? (size_t)s.pitch * s.ny * 4 * sizeof(double) : 0;
+    if (hipMalloc(&saved[k].buf, bytes + mbytes) != hipSuccess) {
       release();
       return fail_with(LBM_ERR_NOMEM, "autotune: no room for a state snapshot");
     }
     HIP_TRY(hipMemcpyAsync(saved[k].buf, s.state[s.cur], bytes, hipMemcpyDeviceToDevice, b.stream));
+    if (mbytes)
+      HIP_TRY(hipMemcpyAsync((char*)saved[k].buf + bytes, s.mom[s.cur], mbytes, hipMemcpyDeviceToDevice, b.stream));
     HIP_TRY(hipMemcpyAsync(&saved[k].fail, s.fail, sizeof(int), hipMemcpyDeviceToHost, b.stream));
     saved[k].cur = s.cur;
     saved[k].step = s.step;
@@ -1094,10 +1223,13 @@ extern "C" int lbm_batch_autotune(lbm_batch* hb, int32_t steps, int32_t* shape_o
   for (size_t k = 0; k < b.members.size(); k++) {
     Solver& s = *b.members[k];
     const size_t bytes = (size_t)s.pitch * s.ny * kPlanes * es;
+    const size_t mbytes = s.mom[0] ? (size_t)s.pitch * s.ny * 4 * sizeof(double) : 0;
     s.cur = saved[k].cur;
     s.step = saved[k].step;
     s.failed = saved[k].failed;
     HIP_TRY(hipMemcpyAsync(s.state[s.cur], saved[k].buf, bytes, hipMemcpyDeviceToDevice, b.stream));
+    if (mbytes)
+      HIP_TRY(hipMemcpyAsync(s.mom[s.cur], (char*)saved[k].buf + bytes, mbytes, hipMemcpyDeviceToDevice, b.stream));
     HIP_TRY(hipMemcpyAsync(s.fail, &saved[k].fail, sizeof(int), hipMemcpyHostToDevice, b.stream));
     HIP_TRY(hipStreamSynchronize(b.stream));
     const int st = upload_inst(s);

@@ -58,6 +58,8 @@ struct Solver {
   void* f2 = nullptr;
   void* f3 = nullptr;
   void* dflux = nullptr;
+  double* mom[2] = {nullptr, nullptr};  // FP32 storage: conserved moments of state[k] (4 planes)
+  double ref[kPlanes] = {};             // FP32 storage: reference equilibrium (deviation origin)
   uint8_t* ctype = nullptr;
   double* alpha = nullptr;
   int* fail = nullptr;
