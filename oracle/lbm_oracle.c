@@ -586,7 +586,13 @@ static void step_rk3(orc* o) {
         o->heq3[k] = eqh[i];
         const double fv = o->f[k], f2 = o->fr2[k], fe2 = o->feq1[k], fe3 = eqf[i];
         const double dF2 = net_flux(o, o->flx_f2, x, y, i);
+#if defined(ORC_PLANT_RK) && ORC_PLANT_RK == 2
+        /* planted mistake, built only by the pin test: stage 3 without the
+         * stage-2 flux */
+        o->fr3[k] = (fv + (dt / (2 * tf2)) * (fe2 - f2) + (dt / (2 * m.tf)) * fe3) / (1 + dt / (2 * m.tf));
+#else
         o->fr3[k] = (fv - dF2 + (dt / (2 * tf2)) * (fe2 - f2) + (dt / (2 * m.tf)) * fe3) / (1 + dt / (2 * m.tf));
+#endif
         const double hv = o->h[k], h2 = o->hr2[k], he2 = o->heq1[k], he3 = eqh[i], f3 = o->fr3[k];
         const double eu2 = dot2(o->ex[i], o->ey[i], o->ux1[c], o->uy1[c]);
         const double eu3 = dot2(o->ex[i], o->ey[i], m.ux, m.uy);
@@ -612,14 +618,30 @@ static void step_rk3(orc* o) {
         const double fv = o->f[k], dF2 = o->dF2[k], fe2 = o->feq1[k], f2 = o->fr2[k];
         const double fe3 = o->feq3[k], f3 = o->fr3[k];
         const double dF3 = net_flux(o, o->flx_f3, x, y, i);
+#if defined(ORC_PLANT_RK) && ORC_PLANT_RK == 1
+        /* planted mistake, built only by the pin test: the final combination
+         * takes the whole stage-2 flux instead of the mean of the two */
+        o->f[k] = fv - dF2 + (dt / 2.0) * ((fe2 - f2) / tf2 + (fe3 - f3) / tf3);
+#else
         o->f[k] = fv - 0.5 * (dF2 + dF3) + (dt / 2.0) * ((fe2 - f2) / tf2 + (fe3 - f3) / tf3);
+#endif
         const double hv = o->h[k], dH2 = o->dH2[k], he2 = o->heq1[k], h2 = o->hr2[k];
         const double he3 = o->heq3[k], h3 = o->hr3[k];
         const double eu2 = dot2(o->ex[i], o->ey[i], o->ux1[c], o->uy1[c]);
         const double eu3 = dot2(o->ex[i], o->ey[i], o->ux3[c], o->uy3[c]);
         const double dH3 = net_flux(o, o->flx_h3, x, y, i);
+#if defined(ORC_PLANT_RK) && ORC_PLANT_RK == 1
+        o->h[k] = hv - dH2 + (dt / 2.0) * ((he2 - h2) / th2 + (he3 - h3) / th3) -
+#else
         o->h[k] = hv - 0.5 * (dH2 + dH3) + (dt / 2.0) * ((he2 - h2) / th2 + (he3 - h3) / th3) -
+#endif
+#if defined(ORC_PLANT_RK) && ORC_PLANT_RK == 3
+                  /* planted mistake, built only by the pin test: the f-h
+                   * coupling term of the final combination with its sign flipped */
+                  -(dt / 2.0) * ((eu2 / thf2) * (fe2 - f2) + (eu3 / thf3) * (fe3 - f3));
+#else
                   (dt / 2.0) * ((eu2 / thf2) * (fe2 - f2) + (eu3 / thf3) * (fe3 - f3));
+#endif
       }
     }
 
@@ -734,7 +756,12 @@ static int run_one(orc* o, int want_res, double* res) {
   if (want_res) {
     double s = 0;
     for (size_t k = 0; k < n; k++) s += (o->T[k] - o->res_copy[k]) * (o->T[k] - o->res_copy[k]);
+#if defined(ORC_PLANT_RES) && ORC_PLANT_RES == 1
+    /* planted mistake, built only by the pin test: the mean over one row */
+    *res = sqrt(s / (double)o->nx);
+#else
     *res = sqrt(s / (double)n);
+#endif
   }
   return o->failed;
 }

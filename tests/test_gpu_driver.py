@@ -52,3 +52,32 @@ def test_driver_failure_stop(gpu_lib, tmp_path):
     c.props["pressure"][:] = 0.0
     summary, _ = _run(c, tmp_path)
     assert summary["stop"] == 1 and summary["step"] == 1
+
+
+@pytest.mark.parametrize("name", ["couette", "riemann"])
+def test_residual_matches_oracle(gpu_lib, name):
+    """lbm_step's residual (RMS change of T over the last step, reduced on
+    the device from the two state buffers) against the oracle's runOne
+    residual (cpuLBM.cpp:2322-2331) after 1, 5 and 25 steps. The residual is
+    a difference of two states that each agree to ~1e-13, so the bar is on
+    that scale: |gpu - oracle| <= 1e-12 * rms(T) + 1e-9 * residual."""
+    from pyoracle import Oracle
+    c = cases.couette(40, u_wall=0.15, steps=25) if name == "couette" else cases.riemann(48, steps=25)
+    s = gpu_lib.Solver(c, "fp64")
+    o = Oracle(c)
+    try:
+        done = 0
+        for n in (1, 5, 25):
+            st, r = s.step(n - done, want_residual=True)
+            so, ro = o.step(n - done, want_res=True)
+            done = n
+            assert st == 0 and so == 0
+            T = o.get_macro()["T"]
+            bar = 1e-12 * float(np.sqrt(np.mean(T * T))) + 1e-9 * abs(ro)
+            print(f"[residual] {name} step {n}: gpu {r:.15e} oracle {ro:.15e} |diff| {abs(r - ro):.1e} "
+                  f"(bar {bar:.1e})", flush=True)
+            assert abs(r - ro) <= bar
+            assert r > 0
+    finally:
+        s.close()
+        o.close()

@@ -475,17 +475,17 @@ def test_spacing_conserves_area_weighted_totals(rk, fm):
     assert abs(w - w0) / w > 1e-12
 
 
-def _planted_oracle(tmp_path, plant):
-    """The oracle built with a planted spacing mistake (ORC_PLANT_SPACING),
-    loaded through a private copy of the pyoracle module."""
+def _planted_oracle(tmp_path, plant, macro="ORC_PLANT_SPACING"):
+    """The oracle built with a planted mistake (`macro` = `plant`), loaded
+    through a private copy of the pyoracle module."""
     import importlib.util
     import subprocess
     here = os.path.dirname(pyoracle.__file__)
-    so = tmp_path / f"liblbm_oracle_plant{plant}.so"
+    so = tmp_path / f"liblbm_oracle_{macro}{plant}.so"
     subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c11",
-                    "-D_POSIX_C_SOURCE=199309L", f"-DORC_PLANT_SPACING={plant}", "-shared", "-o", str(so),
+                    "-D_POSIX_C_SOURCE=199309L", f"-D{macro}={plant}", "-shared", "-o", str(so),
                     os.path.join(here, "lbm_oracle.c"), "-lm"], check=True)
-    spec = importlib.util.spec_from_file_location(f"pyoracle_plant{plant}", pyoracle.__file__)
+    spec = importlib.util.spec_from_file_location(f"pyoracle_{macro}{plant}", pyoracle.__file__)
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
     mod.LIB_PATH = str(so)
@@ -527,3 +527,158 @@ def test_spacing_time_step_and_refusals():
     with pytest.raises(ValueError):
         ow.set_spacing(np.full(wall.nx, wall.params["dx"]), np.full(wall.ny, wall.params["dy"]))
     ow.close()
+
+
+# --------------------------------------------------------------------------
+# A11 stepper, the flux half: with the relaxation switched off (mu -> inf,
+# so dt/tau ~ 1e-14) IMEX-SSP2(3,2,2) reduces to its explicit part, Heun's
+# method on the net flux: f* = f - F(f), f_new = f - (F(f) + F(f*))/2
+# (reference RK3 stages, cpuLBM.cpp:1945, 1970, 2022, 2039). The Heun step is
+# built here from the oracle's own face fluxes and cFLUX, which the tests
+# above pin separately. And with the relaxation on, the coupled stepper is
+# second order in time at fixed dx.
+# --------------------------------------------------------------------------
+def _net_flux(o, flux, nx, ny):
+    out = np.zeros((nx, ny, 13))
+    for x in range(nx):
+        for y in range(ny):
+            for i in range(13):
+                out[x, y, i] = o.cflux(flux, x, y, i)
+    return out
+
+
+def _heun_gap(mod):
+    c = cases.shear_wave(16, u0=0.15, tau=1e12, steps=1)
+    o = mod.Oracle(c)
+    f, h = o.get_dist()
+    m = 1 + 0.05 * np.outer(np.sin(2 * np.pi * np.arange(16) / 16), np.cos(2 * np.pi * np.arange(16) / 16))
+    f0, h0 = f * m[:, :, None], h * m[:, :, None]
+    o.set_dist(f0, h0)
+    ff, hf = o.boundary_fluxes(f0, h0)
+    F1, H1 = _net_flux(o, ff, 16, 16), _net_flux(o, hf, 16, 16)
+    ff2, hf2 = o.boundary_fluxes(f0 - F1, h0 - H1)
+    F2, H2 = _net_flux(o, ff2, 16, 16), _net_flux(o, hf2, 16, 16)
+    fh, hh = f0 - 0.5 * (F1 + F2), h0 - 0.5 * (H1 + H2)
+    o.step(1)
+    fo, ho = o.get_dist()
+    o.close()
+    return (float(np.linalg.norm(fo - fh) / np.linalg.norm(fh - f0)),
+            float(np.linalg.norm(ho - hh) / np.linalg.norm(hh - h0)))
+
+
+def test_stepper_without_relaxation_is_heun():
+    gf, gh = _heun_gap(pyoracle)
+    assert gf <= 1e-12 and gh <= 1e-12, (gf, gh)
+
+
+@pytest.mark.parametrize("plant", [1, 2], ids=["final_weight", "stage3_drops_dF2"])
+def test_heun_pin_catches_planted_flux_weights(tmp_path, plant):
+    """Flux-weight mistakes that the conservation, fixed-point and symmetry
+    pins let through: the final combination taking -dF2 instead of
+    -(dF2 + dF3)/2, and stage 3 without dF2."""
+    gf, gh = _heun_gap(_planted_oracle(tmp_path, plant, "ORC_PLANT_RK"))
+    assert max(gf, gh) > 1e-3, (gf, gh)
+
+
+def _time_order_ratios(mod):
+    """Error ratios between successive halvings of dt (CFL 0.4 -> 0.05 at
+    fixed dx) at t = 0.1, against a run at CFL 0.0125, on a doubly periodic
+    wave modulated along both axes with tau = 0.05 (dt/tau up to 0.125)."""
+    out = {}
+    for cfl in (0.4, 0.2, 0.1, 0.05, 0.0125):
+        c = cases.shear_wave(32, u0=0.15, tau=0.05, steps=1)
+        c.params["cfl"] = cfl
+        o = mod.Oracle(c)
+        f, h = o.get_dist()
+        m = 1 + 0.05 * np.outer(np.sin(2 * np.pi * np.arange(32) / 32), np.cos(2 * np.pi * np.arange(32) / 32))
+        o.set_dist(f * m[:, :, None], h * m[:, :, None])
+        o.step(int(round(0.1 / o.consts()["dt"])))
+        out[cfl] = np.concatenate([a.ravel() for a in o.get_dist()])
+        o.close()
+    errs = [np.linalg.norm(out[k] - out[0.0125]) for k in (0.4, 0.2, 0.1, 0.05)]
+    return [float(errs[i] / errs[i + 1]) for i in range(3)]
+
+
+def test_stepper_second_order_in_time():
+    r = _time_order_ratios(pyoracle)
+    assert all(3.5 < x < 4.6 for x in r), r
+
+
+@pytest.mark.parametrize("plant", [1, 2], ids=["final_weight", "stage3_drops_dF2"])
+def test_time_order_catches_planted_flux_weights(tmp_path, plant):
+    r = _time_order_ratios(_planted_oracle(tmp_path, plant, "ORC_PLANT_RK"))
+    assert max(r) < 3.0, r
+
+
+# --------------------------------------------------------------------------
+# Temperature half of the thermal Couette closed form
+# (Items/11.04.18 - Couette flow check for macrossan/couetteConstT.m:1-30):
+# at steady state T = T0 + C s^2 + E across the gap, C = -mu A^2/(2k). With
+# the lattice temperature of the reference deck (Tc = 2.5 T) the profile is
+# free of the period-4 standing wiggle the scheme shows at Tc = T.
+# --------------------------------------------------------------------------
+def _couette_temperature(mod, u_wall):
+    c = cases.couette(32, nx=4, u_wall=u_wall, tau=0.05, cfl=0.5)
+    c.params["Tc_Tref"] = 2.5
+    o = mod.Oracle(c)
+    st = o.step(int(60.0 / o.consts()["dt"]))
+    T = o.get_macro()["T"][0, 1:c.ny - 1]
+    o.close()
+    ys = np.arange(1, c.ny - 1)
+    _, _, prm = cases.couette_analytic(c, ys)
+    s = (ys - 0.5) / (c.ny - 2) - 0.5
+    p = np.polyfit(s, T, 2)
+    rise = T.max() - T.min()
+    return dict(status=st, curvature=float(p[0]), C=prm["C"],
+                fit_residual=float(np.sqrt(np.mean((T - np.polyval(p, s)) ** 2)) / rise),
+                asymmetry=float(np.max(np.abs(T - T[::-1])) / rise))
+
+
+def _temperature_checks(mod):
+    a, b = _couette_temperature(mod, 0.1), _couette_temperature(mod, 0.2)
+    return a, b, [a["status"] == 0 and b["status"] == 0,
+                  abs(a["curvature"] / a["C"] - 1) <= 0.05 and abs(b["curvature"] / b["C"] - 1) <= 0.05,
+                  a["fit_residual"] <= 0.02 and b["fit_residual"] <= 0.02,
+                  a["asymmetry"] <= 0.05 and b["asymmetry"] <= 0.05,
+                  3.8 <= b["curvature"] / a["curvature"] <= 4.2]
+
+
+def test_couette_temperature_profile_is_quadratic():
+    """Steady viscous heating (u_wall 0.1 and 0.2, Kn = 0.063): quadratic in
+    s within 2 % of the rise, symmetric about the mid-plane, curvature within
+    5 % of the closed form's C (measured 2 %), and scaling with u_wall^2."""
+    a, b, ok = _temperature_checks(pyoracle)
+    assert all(ok), (a, b, ok)
+
+
+@pytest.mark.parametrize("plant", [3, 1], ids=["coupling_sign", "final_weight"])
+def test_temperature_pin_catches_planted_mistakes(tmp_path, plant):
+    a, b, ok = _temperature_checks(_planted_oracle(tmp_path, plant, "ORC_PLANT_RK"))
+    assert not all(ok), (a, b)
+
+
+# --------------------------------------------------------------------------
+# A12 residual (runOne, cpuLBM.cpp:2322-2331): RMS over all nx*ny cells of
+# the change of T over the step.
+# --------------------------------------------------------------------------
+def _residual_gap(mod):
+    c = cases.couette(12, nx=10, u_wall=0.15, steps=5)
+    o = mod.Oracle(c)
+    o.step(3)
+    T0 = o.get_macro()["T"]
+    st, r = o.step(1, want_res=True)
+    T1 = o.get_macro()["T"]
+    o.close()
+    expect = float(np.sqrt(np.mean((T1 - T0) ** 2)))
+    return st, r, expect
+
+
+def test_residual_is_rms_temperature_change():
+    st, r, expect = _residual_gap(pyoracle)
+    assert st == 0 and r > 0
+    assert r == pytest.approx(expect, rel=1e-12)
+
+
+def test_residual_pin_catches_planted_mean(tmp_path):
+    st, r, expect = _residual_gap(_planted_oracle(tmp_path, 1, "ORC_PLANT_RES"))
+    assert r != pytest.approx(expect, rel=1e-6)

@@ -1543,6 +1543,66 @@ void launch_totals(const InstDev* insts, int inst, int par, double* scratch, lon
 int totals_scratch_doubles() { return 4 * kRedBlocks + 4; }
 
 // ---------------------------------------------------------------------------
+// Residual of the last step (reference runOne, cpuLBM.cpp:2322-2331): RMS
+// over all nx*ny cells of the change of T between the previous state (the
+// other state buffer, which a step leaves untouched) and the current one.
+// Deterministic two-level FP64 reduction: one partial sum per grid row
+// (rows[y], wavefront shuffles within the row's block), then one block adds
+// the rows in a fixed order and writes sqrt(sum / (nx ny)) to rows[ny].
+// ---------------------------------------------------------------------------
+template <typename TS>
+__device__ __forceinline__ double cell_T(const InstDev& in, int k, long long o) {
+  CellState<TS> cs;
+  load_state(in, state_buf<TS>(in, k), in.mom[k], o, cs);
+  Macro m;
+  finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
+  return m.T;
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_resid_rows(const InstDev* insts, int inst, int par, double* rows) {
+  __shared__ double lds4[4];
+  const InstDev& in = insts[inst];
+  const int now = in.parity0 ^ par, prev = 1 ^ now;
+  for (int y = blockIdx.x; y < in.ny; y += gridDim.x) {
+    double acc = 0;
+    for (int x = threadIdx.x; x < in.nx; x += blockDim.x) {
+      const long long o = (long long)y * in.pitch + x;
+      const double d = cell_T<TS>(in, now, o) - cell_T<TS>(in, prev, o);
+      acc += d * d;
+    }
+    const double b = block_sum(acc, lds4);
+    if (threadIdx.x == 0) rows[y] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_resid_final(double* rows, int ny, double cells) {
+  __shared__ double lds4[4];
+  double v = 0;
+  for (int j = threadIdx.x; j < ny; j += blockDim.x) v += rows[j];
+  const double b = block_sum(v, lds4);
+  if (threadIdx.x == 0) rows[ny] = sqrt(b / cells);
+}
+
+// Failure steps of the members of a launch group gathered into one array,
+// so that the host reads them with a single copy.
+__global__ void k_gather_fail(const InstDev* insts, int n, int* out) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = *insts[k].fail;
+}
+
+void launch_gather_fail(const InstDev* insts, int n, int* out, hipStream_t s) {
+  const int blocks = (n + 255) / 256 < 64 ? (n + 255) / 256 : 64;
+  k_gather_fail<<<blocks, 256, 0, s>>>(insts, n, out);
+}
+
+template <typename TS>
+void launch_residual(const InstDev* insts, int inst, int par, double* rows, int nx, int ny, hipStream_t s) {
+  const int blocks = ny < 2048 ? ny : 2048;
+  k_resid_rows<TS><<<blocks, 256, 0, s>>>(insts, inst, par, rows);
+  k_resid_final<<<1, 256, 0, s>>>(rows, ny, (double)nx * (double)ny);
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------
 void flux_tile_dims(int shape, int* w, int* h) {
@@ -1722,5 +1782,7 @@ template void launch_macro<double>(const InstDev*, int, int, double*, long long,
 template void launch_macro<float>(const InstDev*, int, int, double*, long long, hipStream_t);
 template void launch_totals<double>(const InstDev*, int, int, double*, long long, hipStream_t);
 template void launch_totals<float>(const InstDev*, int, int, double*, long long, hipStream_t);
+template void launch_residual<double>(const InstDev*, int, int, double*, int, int, hipStream_t);
+template void launch_residual<float>(const InstDev*, int, int, double*, int, int, hipStream_t);
 
 }  // namespace uqlbm

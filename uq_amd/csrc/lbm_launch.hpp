@@ -109,4 +109,13 @@ template <typename TS>
 void launch_totals(const InstDev* insts, int inst, int par, double* scratch, long long cells, hipStream_t s);
 int totals_scratch_doubles();
 
+// Copies *insts[k].fail into out[k] for the n members of a launch group.
+void launch_gather_fail(const InstDev* insts, int n, int* out, hipStream_t s);
+
+// Residual of the last step of one instance (RMS change of T over all
+// cells between the other state buffer and state `par`): rows holds ny + 1
+// doubles, per-row sums then the residual in rows[ny].
+template <typename TS>
+void launch_residual(const InstDev* insts, int inst, int par, double* rows, int nx, int ny, hipStream_t s);
+
 }  // namespace uqlbm

@@ -341,7 +341,8 @@ Solver::~Solver() {
   if (graphs) graphs->clear();
   for (void* p : {state[0], state[1], f2, f3, (void*)mom[0], (void*)mom[1], (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
                   (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
-                  (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev})
+                  (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev, (void*)resid_dev,
+                  (void*)fails_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -557,6 +558,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   HIP_TRY(hipMalloc((void**)&s->inst_dev, sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&s->macro_dev, plane * 5 * sizeof(double)));
   HIP_TRY(hipMalloc((void**)&s->step_ctr, sizeof(int)));
+  HIP_TRY(hipMalloc((void**)&s->fails_dev, sizeof(int)));
   s->device_bytes = solver_footprint(nx, ny, precision);
 
   std::vector<uint8_t> ct(plane, 2);
@@ -719,21 +721,26 @@ static void record_timing(double total_ms, KernelTimer* tm) {
 }
 
 // Advance every member of a group by n steps with batched launches: upload
-// the instance table (current buffer, step base), launch, synchronise, then
+// the instance table (current buffer, step base), launch, gather every
+// member's failure step into fails_dev and read them with one copy, then
 // account each member's steps, stopping a member at its failing step
-// (reference main loop stop on runOne failure, main.cpp:21,56,73).
+// (reference main loop stop on runOne failure, main.cpp:21,56,73). With
+// `graphs` null the kernels are launched directly; with `timed` the call's
+// device time is recorded (lbm_get_timing).
 int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, const StepLaunch& L, int precision,
-              hipStream_t stream, int n, GraphCache* graphs) {
+                     hipStream_t stream, int n, GraphCache* graphs, int* fails_dev, bool timed) {
   static const int kFirstStep = 1;
   std::vector<InstDev> table(members.size());
   for (size_t k = 0; k < members.size(); k++) fill_inst(*members[k], table[k]);
   HIP_TRY(hipMemcpyAsync(table_dev, table.data(), table.size() * sizeof(InstDev), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(L.step_ctr, &kFirstStep, sizeof(int), hipMemcpyHostToDevice, stream));
-  KernelTimer* tm = active_timer();
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
+  KernelTimer* tm = timed ? active_timer() : nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, stream));
+  }
   if (graphs && g_graphs_enabled && !tm) {
     const int st = replay_steps(*graphs, L, precision, stream, n);
     if (st != LBM_OK) {
@@ -743,21 +750,25 @@ int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, co
   } else {
     launch_steps(L, precision, stream, n, tm);
   }
-  HIP_TRY(hipEventRecord(e1, stream));
+  if (timed) HIP_TRY(hipEventRecord(e1, stream));
+  std::vector<int> fail_step(members.size(), INT_MAX);
+  launch_gather_fail(table_dev, (int)members.size(), fails_dev, stream);
+  HIP_TRY(hipMemcpyAsync(fail_step.data(), fails_dev, fail_step.size() * sizeof(int), hipMemcpyDeviceToHost,
+                         stream));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  record_timing(ms, tm);
+  if (timed) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    record_timing(ms, tm);
+  }
   int status = LBM_OK;
-  for (Solver* s : members) {
-    int fail_step = INT_MAX;
-    HIP_TRY(hipMemcpyAsync(&fail_step, s->fail, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    if (fail_step != INT_MAX) {
-      const int taken = std::max(0, fail_step - s->step);
+  for (size_t k = 0; k < members.size(); k++) {
+    Solver* s = members[k];
+    if (fail_step[k] != INT_MAX) {
+      const int taken = std::max(0, fail_step[k] - s->step);
       s->cur ^= taken & 1;
       s->step += taken;
       s->failed = true;
@@ -770,6 +781,17 @@ int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, co
   return status;
 }
 
+// Residual of the solver's last step on the device (cpuLBM.cpp:2322-2331):
+// `rows` holds ny + 1 doubles; the result is copied to *out.
+static int device_residual(Solver& s, double* rows, double* out) {
+  if (s.precision == LBM_PREC_FP64) launch_residual<double>(s.inst_dev, 0, 0, rows, s.nx, s.ny, s.stream);
+  else launch_residual<float>(s.inst_dev, 0, 0, rows, s.nx, s.ny, s.stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, rows + s.ny, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return LBM_OK;
+}
+
 extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, double* residual_out) {
   if (!h) return fail_with(LBM_ERR_ARG, "null solver");
   Solver& s = *reinterpret_cast<Solver*>(h);
@@ -780,38 +802,16 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   std::vector<Solver*> one{&s};
-  int st = LBM_OK;
-  if (want_residual && nsteps > 1) {
-    st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps - 1, s.graphs.get());
-    if (st != LBM_OK) {
-      (void)upload_inst(s);
-      return st;
-    }
-  }
-  std::vector<double> before;
-  if (want_residual) {
-    st = upload_inst(s);
-    if (st == LBM_OK) st = macro_into(s, before);
-    if (st != LBM_OK) return st;
-  }
-  st = run_group(one, s.inst_dev, L, s.precision, s.stream, want_residual ? 1 : nsteps, s.graphs.get());
-  int st2 = upload_inst(s);
+  const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, s.graphs.get(), s.fails_dev, true);
+  const int st2 = upload_inst(s);
   if (st < 0) return st;
   if (st2 != LBM_OK) return st2;
   if (want_residual && residual_out) {
-    // residual: RMS change of T over the last step (cpuLBM.cpp:2322-2331)
-    std::vector<double> after;
-    st2 = macro_into(s, after);
-    if (st2 != LBM_OK) return st2;
-    const size_t n = (size_t)s.pitch * s.ny;
-    double sum = 0;
-    for (int y = 0; y < s.ny; y++)
-      for (int x = 0; x < s.nx; x++) {
-        const size_t o = 4 * n + (size_t)y * s.pitch + x;
-        const double d = after[o] - before[o];
-        sum += d * d;
-      }
-    *residual_out = std::sqrt(sum / ((double)s.nx * s.ny));
+    // the previous state is still in the other state buffer: the RMS change
+    // of T over the last step is reduced on the device
+    if (!s.resid_dev) HIP_TRY(hipMalloc((void**)&s.resid_dev, ((size_t)s.ny + 1) * sizeof(double)));
+    const int st3 = device_residual(s, s.resid_dev, residual_out);
+    if (st3 != LBM_OK) return st3;
   }
   return st;
 }
@@ -1110,6 +1110,7 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   HIP_TRY(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   HIP_TRY(hipMalloc((void**)&b.inst_dev, b.members.size() * sizeof(InstDev)));
   HIP_TRY(hipMalloc((void**)&b.step_ctr, sizeof(int)));
+  HIP_TRY(hipMalloc((void**)&b.fails_dev, b.members.size() * sizeof(int)));
   int st = batch_set_shape(b, default_flux_shape(b.precision));
   if (st != LBM_OK) return st;
   if (!walls.empty()) {
@@ -1124,7 +1125,8 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
 
 uqlbm::Batch::~Batch() {
   if (graphs) graphs->clear();
-  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr})
+  for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr,
+                  (void*)fails_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -1132,7 +1134,7 @@ uqlbm::Batch::~Batch() {
 int uqlbm::batch_step(Batch& b, int nsteps) {
   StepLaunch L{b.inst_dev,    b.tiles_dev, b.ntiles,    b.walls_dev, b.nwalls,
                b.ptiles_dev, b.nptiles,   b.rk_method, b.step_ctr,  b.flux_shape};
-  int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get());
+  int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get(), b.fails_dev, true);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);
     if (st2 != LBM_OK) return st2;

@@ -78,6 +78,8 @@ struct Solver {
   std::vector<double> dxs, dys;     // non-uniform spacing (lbm_set_spacing), empty when uniform
   double* spacing_dev = nullptr;    // dt/dx[nx] then dt/dy[ny]
   double* totals_dev = nullptr;     // conserved-totals reduction scratch (allocated on first use)
+  double* resid_dev = nullptr;      // residual row sums, ny + 1 doubles (allocated on first use)
+  int* fails_dev = nullptr;         // failure step gathered for the host (one int)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   size_t device_bytes = 0;
@@ -102,6 +104,7 @@ struct Batch {
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
   int* step_ctr = nullptr;
+  int* fails_dev = nullptr;         // members' failure steps gathered for one host copy
   int flux_shape = 0;
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
@@ -117,6 +120,7 @@ struct Batch {
     walls_dev = o.walls_dev;
     ptiles_dev = o.ptiles_dev;
     step_ctr = o.step_ctr;
+    fails_dev = o.fails_dev;
     flux_shape = o.flux_shape;
     graphs = std::move(o.graphs);
     ntiles = o.ntiles;
@@ -128,6 +132,7 @@ struct Batch {
     o.walls_dev = nullptr;
     o.ptiles_dev = nullptr;
     o.step_ctr = nullptr;
+    o.fails_dev = nullptr;
     return *this;
   }
   ~Batch();
@@ -142,6 +147,6 @@ int upload_inst(Solver& s);
 int batch_finish(Batch& b, lbm_batch** out);
 int batch_step(Batch& b, int nsteps);
 int run_group(const std::vector<Solver*>& members, InstDev* table_dev, const StepLaunch& L, int precision,
-              hipStream_t stream, int n, GraphCache* graphs);
+              hipStream_t stream, int n, GraphCache* graphs, int* fails_dev, bool timed);
 
 }  // namespace uqlbm

@@ -66,6 +66,7 @@ struct Server {
   int ntiles = 0, nwalls = 0, nptiles = 0;
   bool lists_dirty = true;
   int* step_ctr = nullptr;
+  int* fails_dev = nullptr;  // failure steps of the active requests, gathered for one host copy
   int flux_shape = 0;
   GraphCache graphs;  // step graphs of the current active set (recaptured when it changes)
   ~Server();
@@ -81,7 +82,8 @@ static size_t footprint_of(int nx, int ny, int precision) { return solver_footpr
 Server::~Server() {
   graphs.clear();
   for (auto& a : active) delete a.solver;
-  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr})
+  for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr,
+                  (void*)fails_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -236,6 +238,7 @@ extern "C" int lbm_server_create(int32_t precision, int32_t rk_method, size_t hb
   SV_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
   SV_TRY(hipMalloc((void**)&sv->table_dev, (size_t)max_slots * sizeof(InstDev)));
   SV_TRY(hipMalloc((void**)&sv->step_ctr, sizeof(int)));
+  SV_TRY(hipMalloc((void**)&sv->fails_dev, (size_t)max_slots * sizeof(int)));
   *out = reinterpret_cast<lbm_server*>(sv.release());
   return LBM_OK;
 }
@@ -292,7 +295,7 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
                  sv.ptiles_dev, sv.nptiles,   sv.rk_method, sv.step_ctr,  sv.flux_shape};
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
-    st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round, &sv.graphs);
+    st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round, &sv.graphs, sv.fails_dev, true);
     if (st < 0) return st;
     if (st == LBM_FAILED) status = LBM_FAILED;
     sv.lups += cells * round;
