@@ -135,6 +135,22 @@ int lbm_init(lbm_solver* s);
 int lbm_step(lbm_solver* s, int32_t nsteps, int32_t want_residual,
              double* residual_out);
 
+/*
+ * lbm_step without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. Same arguments, results and
+ * status as lbm_step, bit for bit; the kernels are launched directly
+ * (no graph capture or instantiation, no events) and the residual's
+ * per-row sums live in a fixed device array of the library instead of a
+ * buffer the solver allocates on first use. Limit: with want_residual != 0
+ * the grid may have at most 32768 rows (ny); larger grids get LBM_ERR_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other host threads, other streams) race and must be serialised by
+ * the caller.
+ */
+int lbm_step_noalloc(lbm_solver* s, int32_t nsteps, int32_t want_residual,
+                     double* residual_out);
+
 /* Macroscopic output in [x][y] layout (the public rho_G, u_G, p_G, T_G of
  * cpuLBM.h:93-96, split into ux/uy). Any pointer may be NULL. */
 int lbm_get_macro(lbm_solver* s, double* rho, double* ux, double* uy,

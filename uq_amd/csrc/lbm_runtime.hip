@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -811,6 +812,70 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
     // of T over the last step is reduced on the device
     if (!s.resid_dev) HIP_TRY(hipMalloc((void**)&s.resid_dev, ((size_t)s.ny + 1) * sizeof(double)));
     const int st3 = device_residual(s, s.resid_dev, residual_out);
+    if (st3 != LBM_OK) return st3;
+  }
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Stepping without allocation (lbm_step_noalloc): the residual's row sums
+// live in a file-scope device array instead of the solver's lazily
+// allocated buffer, the kernels are launched directly (no graph capture or
+// instantiation) and no events are created. Host code never passes the
+// array's name to a kernel: its device address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and that pointer is used.
+// Every call on one GPU shares that GPU's array; calls that overlap on one
+// GPU race for it. Nothing in it needs clearing: every call writes each row
+// sum it later reads.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxRows = 32768;  // grid rows (ny) the array serves: 256 KiB of doubles
+static __device__ __align__(256) double scratch[kNoAllocMaxRows + 1];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_on_current_device(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail_with(LBM_ERR_ARG, "device %d beyond the supported %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail_with(LBM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = reinterpret_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM_OK;
+}
+
+extern "C" int lbm_step_noalloc(lbm_solver* h, int32_t nsteps, int32_t want_residual, double* residual_out) {
+  if (!h) return fail_with(LBM_ERR_ARG, "null solver");
+  Solver& s = *reinterpret_cast<Solver*>(h);
+  if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_step_noalloc before lbm_init");
+  if (nsteps < 0) return fail_with(LBM_ERR_ARG, "negative step count");
+  if (want_residual && s.ny > kNoAllocMaxRows)
+    return fail_with(LBM_ERR_ARG, "lbm_step_noalloc: the residual serves grids of at most %d rows, this one has %d",
+                     kNoAllocMaxRows, s.ny);
+  double* rows = nullptr;
+  if (want_residual) {
+    const int st = scratch_on_current_device(&rows);
+    if (st != LBM_OK) return st;
+  }
+  if (s.failed) return LBM_FAILED;
+  if (nsteps == 0) return LBM_OK;
+  StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
+               s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
+  prepare_launch(s.precision == LBM_PREC_FP64);
+  std::vector<Solver*> one{&s};
+  const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, nullptr, s.fails_dev, false);
+  const int st2 = upload_inst(s);
+  if (st < 0) return st;
+  if (st2 != LBM_OK) return st2;
+  if (want_residual && residual_out) {
+    const int st3 = device_residual(s, rows, residual_out);
     if (st3 != LBM_OK) return st3;
   }
   return st;
