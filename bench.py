@@ -201,22 +201,7 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="headline leg only")
     ap.add_argument("--legs", default="all",
                     help="comma list of legs to run (headline,mixed,plain,riemann,serve,cpu); for profiling")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
-
-    if args.copy_probe_mib:
-        # Calibration leg for rocprofv3 FETCH_SIZE/WRITE_SIZE: one device copy
-        # of known size (read N MiB, write N MiB), nothing else.
-        import torch
-        n = args.copy_probe_mib * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_mib": args.copy_probe_mib}))
-        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (one GPU's share of the work); the "
