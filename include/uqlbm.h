@@ -197,6 +197,8 @@ int lbm_get_totals(lbm_solver* s, double* out);
  * CPP/cpuLBM.cpp:360-450), step count and device footprint of a solver. */
 int lbm_get_info(const lbm_solver* s, lbm_info* out);
 
+/* Frees the solver; one still held by a batch is freed when its last batch
+ * is destroyed (the batch keeps stepping it until then). */
 void lbm_destroy(lbm_solver* s);
 
 /*
@@ -266,8 +268,13 @@ int lbm_set_graphs(int32_t enable);
  * admitted (device state allocated and initialised) when a slot is free and
  * its device footprint fits within hbm_budget_bytes. Each tick advances
  * every admitted solve by up to tick_steps steps in batched launches and
- * retires the ones that reached their own step count. Results of a request
- * do not depend on which other requests shared its ticks.
+ * retires the ones that reached their own step count. A request whose deck
+ * has a residual tolerance (lbm_params.tol > 0) also retires once it meets
+ * the reference driver's stop rule (main.cpp:54-110: from step 21 on, the
+ * residual normalised by the largest residual of steps 2..20 is <= tol),
+ * tested at the end of each batched round; with nsteps = 0 it runs until
+ * then. Results of a request do not depend on which other requests shared
+ * its ticks.
  */
 typedef struct lbm_request_result {
   int64_t tag;
@@ -277,6 +284,8 @@ typedef struct lbm_request_result {
   double admit_s;       /* host clock at admission */
   double finish_s;      /* host clock at completion */
   int64_t cells;        /* nx*ny */
+  double residual;      /* normalised residual at completion (requests
+                           with a tolerance; 0 otherwise) */
 } lbm_request_result;
 
 int lbm_server_create(int32_t precision, int32_t rk_method,

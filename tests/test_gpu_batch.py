@@ -125,3 +125,80 @@ def test_server_admits_by_memory_budget(gpu_lib):
             break
     assert seen == 2
     assert len(done) == 6 and all(r["status"] == 0 for r in done)
+
+
+def _normalised_residual_alone(lbm, case, steps):
+    """Solve `case` alone for `steps` steps and apply the reference driver's
+    normalisation (main.cpp:54-110): the residual of the last step over the
+    largest residual of steps 2..20."""
+    s = lbm.Solver(case, "mixed")
+    hist = []
+    for k in range(1, 21):
+        _, r = s.step(1, want_residual=True)
+        if k >= 2:
+            hist.append(r)
+    _, r = s.step(steps - 20, want_residual=True)
+    m = s.get_macro()
+    s.close()
+    return r / max(hist), m
+
+
+def test_server_retires_converged_request_early(gpu_lib):
+    """A request whose deck carries a residual tolerance retires at the first
+    round end where the normalised residual is <= tol, well before its step
+    count; its fields equal the same solve run alone for the same number of
+    steps, and the residual it reports is the one a lone solve computes.
+    A run-to-tolerance request (step count 0) and a plain request share the
+    server with it."""
+    lbm = gpu_lib
+    conv = cases.couette(24, nx=16, u_wall=0.1, tau=0.05, steps=4000)
+    conv.params["Tc_Tref"] = 2.5
+    conv.params["tol"] = 2e-2
+    open_ended = cases.couette(20, nx=8, u_wall=0.08, tau=0.05, steps=0)
+    open_ended.params["Tc_Tref"] = 2.5
+    open_ended.params["tol"] = 5e-2
+    plain = cases.shear_wave(32, u0=0.1, steps=300)
+    sv = lbm.Server(precision="mixed", hbm_budget_bytes=1 << 34, max_slots=8, keep_results=True)
+    try:
+        sv.submit(conv, tag=0, nsteps=4000)
+        sv.submit(open_ended, tag=1, nsteps=0)
+        sv.submit(plain, tag=2)
+        out = []
+        for _ in range(400):
+            active, queued = sv.tick(50)
+            out += sv.poll()
+            if not active and not queued:
+                break
+        by = {r["tag"]: r for r in out}
+        assert sorted(by) == [0, 1, 2]
+        assert by[2]["status"] == 0 and by[2]["steps_done"] == 300 and by[2]["residual"] == 0.0
+        for tag, case in ((0, conv), (1, open_ended)):
+            r = by[tag]
+            tol = case.params["tol"]
+            print(f"[serve] request {tag}: retired after {r['steps_done']} steps, normalised residual "
+                  f"{r['residual']:.4e} (tol {tol})", flush=True)
+            assert r["status"] == 0 and 21 <= r["steps_done"] < 4000
+            assert r["residual"] <= tol
+            res_alone, m_alone = _normalised_residual_alone(lbm, case, r["steps_done"])
+            assert res_alone == pytest.approx(r["residual"], rel=1e-9)
+            m = sv.result_macro(tag)
+            for k in ("rho", "ux", "uy", "T", "p"):
+                assert np.array_equal(m[k], m_alone[k]), (tag, k)
+    finally:
+        sv.close()
+
+
+def test_destroying_a_batched_solver_is_deferred(gpu_lib):
+    lbm = gpu_lib
+    a, b = lbm.Solver(cases.couette(32, steps=10)), lbm.Solver(cases.shear_wave(24, steps=10))
+    ref = lbm.Solver(cases.shear_wave(24, steps=10))
+    batch = lbm.Batch([a, b])
+    a.close()                      # freed only when the batch goes
+    assert batch.step(10) == 0
+    ref.step(10)
+    fb, hb = b.get_dist()
+    fr, hr = ref.get_dist()
+    assert np.array_equal(fb, fr) and np.array_equal(hb, hr)
+    batch.close()
+    b.close()
+    ref.close()

@@ -120,6 +120,7 @@ class LbmRequestResult(C.Structure):
         ("admit_s", C.c_double),
         ("finish_s", C.c_double),
         ("cells", C.c_int64),
+        ("residual", C.c_double),
     ]
 
 

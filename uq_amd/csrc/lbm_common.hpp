@@ -76,7 +76,18 @@ struct InstDev {
   // for a uniform grid (rdx, rdy above)
   const double* rdxs;
   const double* rdys;
+  // residual tracking of a served request with a tolerance (the reference
+  // driver's stop rule, main.cpp:54-110): res_hist[k - 2] receives the
+  // residual of global step k for 2 <= k <= 20, res_hist[19] that of global
+  // step res_last (the last step of the current launch sequence); null
+  // when not tracked. ptile0, nptiles: the instance's run in the group's
+  // pointwise tile list.
+  double* res_hist;
+  int res_last;
+  int ptile0, nptiles;
 };
+
+constexpr int kResHist = 20;  // residual history slots per tracked instance
 
 // One tile of one instance: the unit of work of a block.
 struct TileRef {

@@ -1584,6 +1584,55 @@ __global__ __launch_bounds__(256) void k_resid_final(double* rows, int ny, doubl
   if (threadIdx.x == 0) rows[ny] = sqrt(b / cells);
 }
 
+// Per-step residual of the tracked instances of a launch group (served
+// requests with a tolerance), inside the step sequence: after step `step`
+// of the sequence (state parity par -> par ^ 1) each pointwise tile of an
+// instance whose global step is in 2..20 or equals res_last writes its
+// partial sum of (T_new - T_old)^2; one block per instance then adds its
+// tiles in their fixed order.
+template <typename TS>
+__global__ __launch_bounds__(kNT) void k_track_tiles(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                                     const int* stepp, double* partial) {
+  __shared__ double lds4[4];
+  const int step = *stepp;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const InstDev& in = insts[tr.inst];
+    if (!in.res_hist) continue;
+    const int gs = in.step_base + step;
+    if (!((gs >= 2 && gs <= kResHist) || gs == in.res_last) || frozen_by_failure(in, step)) continue;
+    const PointCell c = point_cell(in, tr);
+    double d2 = 0;
+    if (c.active) {
+      const int prev = in.parity0 ^ par, now = 1 ^ prev;
+      const double d = cell_T<TS>(in, now, c.o) - cell_T<TS>(in, prev, c.o);
+      d2 = d * d;
+    }
+    const double b = block_sum(d2, lds4);
+    if (threadIdx.x == 0) partial[t] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_track_sum(const InstDev* insts, int ninst, const int* stepp,
+                                                   const double* partial) {
+  __shared__ double lds4[4];
+  const int step = *stepp;
+  for (int k = blockIdx.x; k < ninst; k += gridDim.x) {
+    const InstDev& in = insts[k];
+    if (!in.res_hist) continue;
+    const int gs = in.step_base + step;
+    if (!((gs >= 2 && gs <= kResHist) || gs == in.res_last) || frozen_by_failure(in, step)) continue;
+    double v = 0;
+    for (int j = threadIdx.x; j < in.nptiles; j += blockDim.x) v += partial[in.ptile0 + j];
+    const double b = block_sum(v, lds4);
+    if (threadIdx.x == 0) {
+      const double r = sqrt(b / ((double)in.nx * (double)in.ny));
+      if (gs >= 2 && gs <= kResHist) in.res_hist[gs - 2] = r;
+      if (gs == in.res_last) in.res_hist[kResHist - 1] = r;
+    }
+  }
+}
+
 // Failure steps of the members of a launch group gathered into one array,
 // so that the host reads them with a single copy.
 __global__ void k_gather_fail(const InstDev* insts, int n, int* out) {
@@ -1749,6 +1798,13 @@ __global__ void k_tick(int* ctr) {
   if (threadIdx.x == 0) ctr[0] = ctr[0] + 1;
 }
 
+template <typename TS>
+static void launch_track(const StepLaunch& L, hipStream_t s, int par, int ninst) {
+  k_track_tiles<TS><<<L.nptiles < 8192 ? L.nptiles : 8192, kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par,
+                                                                        L.step_ctr, L.res_partial);
+  k_track_sum<<<ninst < 1024 ? ninst : 1024, 256, 0, s>>>(L.insts, ninst, L.step_ctr, L.res_partial);
+}
+
 template <typename TS, typename TC>
 void launch_step(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   switch (L.flux_shape) {
@@ -1758,6 +1814,7 @@ void launch_step(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
     case 4: launch_step_shape<TS, TC, Shape4>(L, s, par, tm); break;
     default: launch_step_shape<TS, TC, Shape0>(L, s, par, tm); break;
   }
+  if (L.res_partial && L.nptiles > 0) launch_track<TS>(L, s, par, L.ninst);
   k_tick<<<1, 64, 0, s>>>(L.step_ctr);
 }
 

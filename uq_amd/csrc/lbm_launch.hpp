@@ -24,12 +24,15 @@ struct StepLaunch {
   int rk_method = 1;                // RKMETHOD shared by every instance of the group
   int* step_ctr = nullptr;          // device step counter: step k of a sequence reads k + 1
   int flux_shape = 0;               // flux tile shape (flux_tile_dims); the tile list is built for it
+  double* res_partial = nullptr;    // per pointwise tile residual partials (nptiles doubles) when
+                                    // some instance tracks its residual; null otherwise
+  int ninst = 0;                    // instances in the table (for the residual tracking)
 };
 
 inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
   return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
          a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
-         a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape;
+         a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape && a.res_partial == b.res_partial && a.ninst == b.ninst;
 }
 
 // Optional per-kernel timing with HIP events recorded on the launch stream

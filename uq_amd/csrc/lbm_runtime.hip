@@ -343,7 +343,7 @@ Solver::~Solver() {
   for (void* p : {state[0], state[1], f2, f3, (void*)mom[0], (void*)mom[1], (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
                   (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
                   (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev, (void*)resid_dev,
-                  (void*)fails_dev})
+                  (void*)fails_dev, (void*)res_hist})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -409,6 +409,10 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.iT_ref_v = s.T_ref_v > 0 ? 1.0 / s.T_ref_v : 0.0;
   d.vexp = 0.5 + s.visc_param;
   d.vS = s.visc_param;
+  d.res_hist = nullptr;  // set per launch group (run_group)
+  d.res_last = 0;
+  d.ptile0 = 0;
+  d.nptiles = s.nptiles;
 }
 
 // Flux tiles of one shape with the per-tile solid flag (the tile's region
@@ -732,7 +736,15 @@ int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, co
                      hipStream_t stream, int n, GraphCache* graphs, int* fails_dev, bool timed) {
   static const int kFirstStep = 1;
   std::vector<InstDev> table(members.size());
-  for (size_t k = 0; k < members.size(); k++) fill_inst(*members[k], table[k]);
+  int ptile0 = 0;
+  for (size_t k = 0; k < members.size(); k++) {
+    fill_inst(*members[k], table[k]);
+    table[k].res_hist = members[k]->res_hist;
+    table[k].res_last = members[k]->step + n;
+    table[k].ptile0 = ptile0;
+    table[k].nptiles = members[k]->nptiles;
+    ptile0 += members[k]->nptiles;
+  }
   HIP_TRY(hipMemcpyAsync(table_dev, table.data(), table.size() * sizeof(InstDev), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(L.step_ctr, &kFirstStep, sizeof(int), hipMemcpyHostToDevice, stream));
   KernelTimer* tm = timed ? active_timer() : nullptr;
@@ -1050,7 +1062,14 @@ extern "C" int lbm_get_info(const lbm_solver* h, lbm_info* out) {
   return LBM_OK;
 }
 
-extern "C" void lbm_destroy(lbm_solver* h) { delete reinterpret_cast<Solver*>(h); }
+// A solver still held by a batch is freed when its last batch is destroyed
+// (the batch keeps stepping it meanwhile); otherwise at once.
+extern "C" void lbm_destroy(lbm_solver* h) {
+  Solver* s = reinterpret_cast<Solver*>(h);
+  if (!s) return;
+  if (s->batch_refs > 0) s->destroy_pending = true;
+  else delete s;
+}
 
 extern "C" int lbm_set_viscosity(lbm_solver* h, int32_t model, double mu_ref, double T_ref, double param) {
   if (!h) return fail_with(LBM_ERR_ARG, "null solver");
@@ -1135,6 +1154,7 @@ extern "C" int lbm_batch_create(lbm_solver* const* members, int32_t n, lbm_batch
     b->precision = s->precision;
     b->rk_method = s->p.rk_method;
     b->members.push_back(s);
+    s->batch_refs++;  // released by ~Batch, also when creation fails below
   }
   return batch_finish(*b, out);
 }
@@ -1190,6 +1210,8 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
 
 uqlbm::Batch::~Batch() {
   if (graphs) graphs->clear();
+  for (Solver* s : members)
+    if (--s->batch_refs == 0 && s->destroy_pending) delete s;
   for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr,
                   (void*)fails_dev})
     if (p) (void)hipFree(p);

@@ -79,6 +79,7 @@ struct Solver {
   double* spacing_dev = nullptr;    // dt/dx[nx] then dt/dy[ny]
   double* totals_dev = nullptr;     // conserved-totals reduction scratch (allocated on first use)
   double* resid_dev = nullptr;      // residual row sums, ny + 1 doubles (allocated on first use)
+  double* res_hist = nullptr;       // served with a tolerance: per-step residual history (kResHist)
   int* fails_dev = nullptr;         // failure step gathered for the host (one int)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
@@ -88,6 +89,8 @@ struct Solver {
   int step = 0;   // steps taken
   bool inited = false;
   bool failed = false;
+  int batch_refs = 0;            // batches holding this solver
+  bool destroy_pending = false;  // lbm_destroy called while batch_refs > 0
 
   ~Solver();
 };

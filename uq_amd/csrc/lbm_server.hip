@@ -34,10 +34,13 @@ struct PendingReq {
 
 struct ActiveReq {
   Solver* solver;
-  int nsteps;
+  int nsteps;              // step count; 0 with a tolerance: run until converged
   int64_t tag;
   double submit_s, admit_s;
   size_t footprint;
+  double tol = 0;          // residual tolerance of the deck (0: none)
+  double residual = 0;     // normalised residual at the last round end
+  bool converged = false;
 };
 
 struct KeptMacro {
@@ -67,6 +70,7 @@ struct Server {
   bool lists_dirty = true;
   int* step_ctr = nullptr;
   int* fails_dev = nullptr;  // failure steps of the active requests, gathered for one host copy
+  double* res_partial = nullptr;  // residual partials per pointwise tile (ptiles_cap doubles)
   int flux_shape = 0;
   GraphCache graphs;  // step graphs of the current active set (recaptured when it changes)
   ~Server();
@@ -83,7 +87,7 @@ Server::~Server() {
   graphs.clear();
   for (auto& a : active) delete a.solver;
   for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr,
-                  (void*)fails_dev})
+                  (void*)fails_dev, (void*)res_partial})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -125,8 +129,11 @@ static int rebuild_lists(Server& sv) {
   }
   if (ptiles.size() > sv.ptiles_cap) {
     if (sv.ptiles_dev) SV_TRY(hipFree(sv.ptiles_dev));
+    if (sv.res_partial) SV_TRY(hipFree(sv.res_partial));
+    sv.res_partial = nullptr;
     sv.ptiles_cap = std::max(ptiles.size(), sv.ptiles_cap * 2);
     SV_TRY(hipMalloc((void**)&sv.ptiles_dev, sv.ptiles_cap * sizeof(TileRef)));
+    SV_TRY(hipMalloc((void**)&sv.res_partial, sv.ptiles_cap * sizeof(double)));
   }
   if (!ptiles.empty())
     SV_TRY(hipMemcpyAsync(sv.ptiles_dev, ptiles.data(), ptiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
@@ -169,7 +176,13 @@ static int admit(Server& sv) {
       continue;
     }
     Solver* s = reinterpret_cast<Solver*>(h);
-    sv.active.push_back(ActiveReq{s, r.nsteps, r.tag, r.submit_s, t, s->device_bytes});
+    ActiveReq a{s, r.nsteps, r.tag, r.submit_s, t, s->device_bytes};
+    a.tol = r.params.tol > 0 ? r.params.tol : 0.0;
+    if (a.tol > 0) {
+      SV_TRY(hipMalloc((void**)&s->res_hist, kResHist * sizeof(double)));
+      SV_TRY(hipMemset(s->res_hist, 0, kResHist * sizeof(double)));
+    }
+    sv.active.push_back(a);
     sv.in_use += s->device_bytes;
     sv.queue.pop_front();
     sv.lists_dirty = true;
@@ -181,7 +194,8 @@ static int retire(Server& sv) {
   std::vector<ActiveReq> keep;
   for (auto& a : sv.active) {
     Solver* s = a.solver;
-    if (s->step < a.nsteps && !s->failed) {
+    const bool reached = a.nsteps > 0 && s->step >= a.nsteps;
+    if (!reached && !a.converged && !s->failed) {
       keep.push_back(a);
       continue;
     }
@@ -193,6 +207,7 @@ static int retire(Server& sv) {
     res.admit_s = a.admit_s;
     res.finish_s = now_s();
     res.cells = (int64_t)s->nx * s->ny;
+    res.residual = a.residual;
     if (sv.keep) {
       // the solver's own instance record still describes admission time
       int su = upload_inst(*s);
@@ -216,6 +231,32 @@ static int retire(Server& sv) {
     sv.lists_dirty = true;
   }
   sv.active.swap(keep);
+  return LBM_OK;
+}
+
+// Tolerance stop of the reference driver (main.cpp:54-110): the residual of
+// steps 2..20 sets the normalisation (their maximum); from step 21 on the
+// solve stops once residual / normalisation <= tol. A served request is
+// tested at the end of each round (its last step's residual), so it stops
+// at the first round end that meets the rule.
+static int check_convergence(Server& sv) {
+  std::vector<std::vector<double>> hist(sv.active.size());
+  for (size_t k = 0; k < sv.active.size(); k++) {
+    if (!(sv.active[k].tol > 0)) continue;
+    hist[k].resize(kResHist);
+    SV_TRY(hipMemcpyAsync(hist[k].data(), sv.active[k].solver->res_hist, kResHist * sizeof(double),
+                          hipMemcpyDeviceToHost, sv.stream));
+  }
+  SV_TRY(hipStreamSynchronize(sv.stream));
+  const int numRes = kResHist;
+  for (size_t k = 0; k < sv.active.size(); k++) {
+    ActiveReq& a = sv.active[k];
+    if (!(a.tol > 0) || a.solver->failed || a.solver->step < numRes + 1) continue;
+    double norm = 0;
+    for (int i = 0; i < numRes - 1; i++) norm = std::max(norm, hist[k][i]);
+    a.residual = norm > 0 ? hist[k][numRes - 1] / norm : hist[k][numRes - 1];
+    if (a.residual <= a.tol) a.converged = true;
+  }
   return LBM_OK;
 }
 
@@ -247,6 +288,11 @@ extern "C" int lbm_server_submit(lbm_server* h, const lbm_params* params, int32_
                                  const int32_t* bnd, const lbm_props* props, int32_t nsteps, int64_t tag) {
   if (!h || !params || !bnd || !props || nx < 4 || ny < 4 || nsteps < 0 || props->num_props < 1)
     return fail_with(LBM_ERR_ARG, "bad request");
+  if (!props->density || !props->pressure || !props->temp || !props->velocity_x || !props->velocity_y ||
+      !props->cell_type)
+    return fail_with(LBM_ERR_ARG, "incomplete property table");
+  if (nsteps == 0 && !(params->tol > 0))
+    return fail_with(LBM_ERR_ARG, "a request needs a step count or a residual tolerance");
   Server& sv = *reinterpret_cast<Server*>(h);
   if (params->rk_method != sv.rk_method)
     return fail_with(LBM_ERR_ARG, "request RKMETHOD %d differs from the server's %d", params->rk_method, sv.rk_method);
@@ -282,8 +328,11 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
   if (st != LBM_OK) return st;
   while (left > 0 && !sv.active.empty()) {
     int round = left;
-    for (auto& a : sv.active) round = std::min(round, a.nsteps - a.solver->step);
+    for (auto& a : sv.active)
+      if (a.nsteps > 0) round = std::min(round, a.nsteps - a.solver->step);
     if (round < 1) round = 1;
+    bool track = false;
+    for (auto& a : sv.active) track |= a.tol > 0;
     if (sv.lists_dirty) {
       st = rebuild_lists(sv);
       if (st != LBM_OK) return st;
@@ -293,6 +342,10 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     for (auto& a : sv.active) members.push_back(a.solver);
     StepLaunch L{sv.table_dev,  sv.tiles_dev, sv.ntiles,     sv.walls_dev, sv.nwalls,
                  sv.ptiles_dev, sv.nptiles,   sv.rk_method, sv.step_ctr,  sv.flux_shape};
+    if (track) {
+      L.res_partial = sv.res_partial;
+      L.ninst = (int)members.size();
+    }
     long long cells = 0;
     for (Solver* s : members) cells += (long long)s->nx * s->ny;
     st = run_group(members, sv.table_dev, L, sv.precision, sv.stream, round, &sv.graphs, sv.fails_dev, true);
@@ -300,6 +353,10 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     if (st == LBM_FAILED) status = LBM_FAILED;
     sv.lups += cells * round;
     left -= round;
+    if (track) {
+      st = check_convergence(sv);
+      if (st != LBM_OK) return st;
+    }
     st = retire(sv);
     if (st != LBM_OK) return st;
     st = admit(sv);
