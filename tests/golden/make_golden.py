@@ -46,16 +46,21 @@ FIXTURES = {
 }
 FIELDS = ("rho", "ux", "uy", "T")
 
-# name -> (builder, args, kwargs, max_steps): decks on which the scheme itself
-# is unstable; the fixture records the step at which the oracle's
-# equilibrium check trips. (Ensemble member 84 with the lattice temperature
-# equal to the flow temperature: a Mach-0.42 Couette deck.)
+# name -> (builder, args, kwargs, max_steps, spread): decks on which the
+# scheme itself is unstable; the fixture records the step at which the
+# oracle's equilibrium check trips and, with spread = k > 0, the steps at
+# which it trips when started from k one-ulp perturbations of the initial
+# state (seeds 1..k): on a shock problem the failure step itself depends on
+# rounding. (Ensemble member 84 with the lattice temperature equal to the
+# flow temperature: a Mach-0.42 Couette deck, uniform along x, whose failure
+# step does not move.)
 FAILURES = {
-    "couette64_uq84_tc1_failure": ("uq_member", [84], {"tc_tref": 1.0}, 5000),
+    "couette64_uq84_tc1_failure": ("uq_member", [84], {"tc_tref": 1.0}, 5000, 0),
     # the four-quadrant Riemann deck of the reference's Lax-Liu generator
-    # (physical air viscosity): stable over 2000 steps at 64^2 and 96^2, it
-    # trips the equilibrium check at 128^2 (and, sooner, on finer grids)
-    "riemann128_failure": ("riemann", [128], {}, 2000),
+    # (physical air viscosity, CFL 0.2 as makeLaxLiu3.py:22): stable over
+    # 2000 steps at 64^2 and 96^2, it trips the equilibrium check at 128^2
+    # near step 1050 (and near there on finer grids too)
+    "riemann128_failure": ("riemann", [128], {}, 2000, 6),
 }
 
 
@@ -158,20 +163,40 @@ def make_jacobs() -> str:
     return path
 
 
-def make_failure(name: str) -> str:
+def _failure_step(job) -> int:
+    """(spec, max_steps, seed) -> the step at which the oracle's check trips
+    (-1 when it does not), from the initial state scaled by (1 +- eps) per
+    value with random signs when seed > 0."""
     from pyoracle import Oracle
+    spec, max_steps, seed = job
+    o = Oracle(build_case(spec))
+    if seed:
+        f, h = o.get_dist()
+        rng = np.random.default_rng(seed)
+        eps = np.finfo(np.float64).eps
+        o.set_dist(f * (1 + eps * rng.choice([-1.0, 1.0], f.shape)),
+                   h * (1 + eps * rng.choice([-1.0, 1.0], h.shape)))
+    st = o.step(max_steps)
+    step = int(o.consts()["step"]) if st else -1
+    o.close()
+    return step
+
+
+def make_failure(name: str) -> str:
+    import multiprocessing as mp
     spec = spec_of(name)
-    max_steps = FAILURES[name][3]
+    max_steps, spread = FAILURES[name][3], FAILURES[name][4]
     case = build_case(spec)
     t = time.time()
-    o = Oracle(case)
-    st = o.step(max_steps)
-    fail_step = int(o.consts()["step"]) if st else -1
-    o.close()
+    jobs = [(spec, max_steps, seed) for seed in range(spread + 1)]
+    with mp.get_context("spawn").Pool(min(len(jobs), os.cpu_count() or 1)) as pool:
+        steps = pool.map(_failure_step, jobs)
+    fail_step = steps[0]
     path = os.path.join(HERE, name + ".npz")
-    np.savez_compressed(path, spec=np.array(spec), max_steps=np.int64(max_steps), fail_step=np.int64(fail_step))
-    print(f"{name}: {case.nx}x{case.ny}, oracle fails at step {fail_step}, {time.time() - t:.1f}s "
-          f"-> {os.path.relpath(path, ROOT)}")
+    np.savez_compressed(path, spec=np.array(spec), max_steps=np.int64(max_steps), fail_step=np.int64(fail_step),
+                        fail_steps_perturbed=np.array(steps[1:], dtype=np.int64))
+    print(f"{name}: {case.nx}x{case.ny}, oracle fails at step {fail_step} (perturbed: {steps[1:]}), "
+          f"{time.time() - t:.1f}s -> {os.path.relpath(path, ROOT)}")
     return path
 
 

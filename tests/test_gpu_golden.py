@@ -45,13 +45,19 @@ def test_gpu_matches_fixture(name, gpu_lib):
 @pytest.mark.parametrize("name", sorted(make_golden.FAILURES))
 def test_gpu_fails_at_oracle_step(name, gpu_lib):
     """A deck on which the scheme itself blows up: the GPU FP64 path trips
-    the equilibrium check at the same step as the oracle (thousands of
-    steps of agreement before it)."""
+    the equilibrium check where the oracle does. On the x-invariant Couette
+    deck that is the same step (thousands of steps of agreement before it);
+    on the Riemann deck the failure step moves with rounding (the oracle
+    from one-ulp perturbations of its initial state fails anywhere in a
+    range of steps), and the GPU must fail within that range, widened by 2 %
+    on either side."""
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     case = make_golden.build_case(str(z["spec"]))
     fail_step = int(z["fail_step"])
-    assert fail_step > 0
-    print(f"[golden] {name} {case.nx}x{case.ny}: oracle fails at step {fail_step}", flush=True)
+    spread = [int(v) for v in z["fail_steps_perturbed"]] if "fail_steps_perturbed" in z.files else []
+    assert fail_step > 0 and all(v > 0 for v in spread)
+    print(f"[golden] {name} {case.nx}x{case.ny}: oracle fails at step {fail_step} (perturbed: {spread})",
+          flush=True)
     s = gpu_lib.Solver(case, "fp64")
     try:
         st = s.step(int(z["max_steps"]))
@@ -59,7 +65,12 @@ def test_gpu_fails_at_oracle_step(name, gpu_lib):
     finally:
         s.close()
     print(f"[golden] {name}: GPU status {st} at step {step}", flush=True)
-    assert st == 1 and step == fail_step
+    assert st == 1
+    if spread:
+        lo, hi = min([fail_step] + spread), max([fail_step] + spread)
+        assert 0.98 * lo <= step <= 1.02 * hi
+    else:
+        assert step == fail_step
 
 
 def test_jacobs_boundary_layer(gpu_lib):

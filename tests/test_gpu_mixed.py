@@ -4,8 +4,10 @@ the ensemble's full 5000 steps.
 The mixed path stores every distribution in FP32 as its deviation from the
 instance's reference equilibrium (the equilibrium at rest at the deck's mean
 density and temperature), keeps each state buffer's conserved moments (mass,
-momentum, energy) in four FP64 planes, forms the pointwise stages in FP64
-and the stencil reconstructions in FP32 with FP64 WENO weight normalisation.
+momentum, energy) in four FP64 planes, evaluates the equilibria in FP32 in
+a regrouped form whose every term is of the deviation's size, the stage
+updates in FP64 (rounded once, on the store) and the stencil
+reconstructions in FP32 with FP64 WENO weight normalisation.
 The FP64 path matches the CPU oracle to 1e-12 (tests/test_gpu_parity.py and
 the batched-member oracle tests), so it stands in for the per-instance FP64
 reference run here.

@@ -142,6 +142,97 @@ __host__ __device__ __forceinline__ void eq_dir(const EqIn<T>& q, T& fe, T& he) 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Equilibria as deviations from the instance's reference equilibrium (the
+// equilibrium at rest at density rho0 and t0 = T0/Tc), for FP32 storage:
+//   feq - ref = rho (P_uv(u, v, t) + P(0, 0, t) - P(0, 0, t0)) + (rho - rho0) P(0, 0, t0)
+// where P_uv collects the terms of the closed form that carry a velocity
+// factor and the rest-state difference is factored by t - t0. Every term
+// is of the size of the deviation, so FP32 evaluation errs relative to the
+// deviation instead of to the full value. Same closed forms as above
+// (reference inlineFunctions.cpp:10-61), regrouped.
+// ---------------------------------------------------------------------------
+struct EqDev {
+  float u, v, t, dt;    // velocity / sqrt(R Tc), T/Tc, T/Tc - t0
+  float tt, t2s;        // t + t0, t^2 + t t0 + t0^2
+  float rho, drho;      // density, density - rho0
+  float K, scale, dscale;  // internal degrees of freedom, rho R Tc, (rho - rho0) R Tc
+};
+
+__device__ __forceinline__ float pf0_uv(float u, float v, float t) {
+  const float u2 = u * u, v2 = v * v;
+  return 0.25f * (u2 * u2 - 5.f * v2 + v2 * v2 + 10.f * t * (u2 + v2) + u2 * (-5.f + 4.f * v2));
+}
+__device__ __forceinline__ float pf1_uv(float a, float b, float t) {
+  const float a2 = a * a, b2 = b * b;
+  return (1.f / 6.f) * (-a * (1.f + a) * (-4.f + a2 + 3.f * b2) - t * (6.f * a + 9.f * a2 + 3.f * b2));
+}
+__device__ __forceinline__ float pf5_uv(float a, float b, float t) {
+  return 0.25f * (t * (b + b * b) + (a + a * a) * (t + b + b * b));
+}
+__device__ __forceinline__ float pf9_uv(float a, float t) {
+  return (1.f / 24.f) * ((-1.f + a) * a * (1.f + a) * (2.f + a) + 6.f * t * a * (1.f + a));
+}
+__device__ __forceinline__ float ph0_uv(float K, float t, float u, float v) {
+  const float u2 = u * u, v2 = v * v, u4 = u2 * u2, v4 = v2 * v2;
+  return (3.f * t * (-40.f * u2 - 5.f * K * u2 + 20.f * u4 + K * u4 + (-5.f * (8.f + K) + 4.f * (15.f + K) * u2) * v2 +
+                     (20.f + K) * v4) +
+          30.f * (t * t) * ((9.f + K) * (u2 + v2)) +
+          3.f * (u2 + v2) * (4.f + u4 - 5.f * v2 + v4 + u2 * (-5.f + 4.f * v2))) *
+         (1.f / 24.f);
+}
+__device__ __forceinline__ float ph1_uv(float K, float t, float a, float b) {
+  const float a2 = a * a, a3 = a2 * a, b2 = b * b;
+  return (-a * (1.f + a) * (a2 + b2) * (-4.f + a2 + 3.f * b2) -
+          t * (a * (-4.f * (4.f + K) - 4.f * (7.f + K) * a + (14.f + K) * a2 + (19.f + K) * a3) +
+               (-4.f + 3.f * a * (10.f + K + (14.f + K) * a)) * b2 + 3.f * (b2 * b2)) -
+          (t * t) * (6.f * a * (6.f + 13.f * a) + 30.f * b2 + K * (6.f * a + 9.f * a2 + 3.f * b2))) *
+         (1.f / 12.f);
+}
+__device__ __forceinline__ float ph5_uv(float K, float t, float a, float b) {
+  const float a2 = a * a, b2 = b * b;
+  return (3.f * a * (1.f + a) * b * (1.f + b) * (a2 + b2) +
+          3.f * (t * t) * ((6.f + K) * a + (9.f + K) * a2 + b * (6.f + K + (9.f + K) * b)) +
+          3.f * t *
+              (a2 * a + a2 * a2 + b2 * b * (1.f + b) + a * b * (6.f + K + (9.f + K) * b) +
+               a2 * b * (9.f + K + (12.f + K) * b))) *
+         (1.f / 24.f);
+}
+__device__ __forceinline__ float ph9_uv(float K, float t, float a, float b) {
+  const float a2 = a * a, b2 = b * b;
+  return (t * a * (-8.f + K * (-1.f + a) * (1.f + a) * (2.f + a) + a * (-7.f + 2.f * a * (11.f + 8.f * a))) +
+          t * (-1.f + 6.f * a * (1.f + a)) * b2 + (-1.f + a) * a * (1.f + a) * (2.f + a) * (a2 + b2) +
+          (t * t) * (36.f * a + 51.f * a2 + 6.f * K * a * (1.f + a) + 3.f * b2)) *
+         (1.f / 48.f);
+}
+
+// f and h equilibrium deviations of direction I; eqref holds the base
+// polynomials at rest at t0 (f per unit density, h per unit scale).
+template <int I>
+__device__ __forceinline__ void eq_dev(const EqDev& q, const float* eqref, float& fe, float& he) {
+  constexpr int base = eq_base(I);
+  const float a = float(eq_sa(I)) * (eq_swap(I) ? q.v : q.u);
+  const float b = float(eq_sb(I)) * (eq_swap(I) ? q.u : q.v);
+  const float K = q.K, t = q.t, dt = q.dt, tt = q.tt, t2s = q.t2s;
+  const float c3 = 16.f + 3.f * K;
+  if constexpr (base == 0) {
+    fe = q.rho * (pf0_uv(q.u, q.v, t) + 2.5f * dt * (tt - 1.f)) + q.drho * eqref[0];
+    he = q.scale * (ph0_uv(K, t, q.u, q.v) +
+                    dt * (10.f * c3 * t2s + 3.f * (8.f + 4.f * K) - 30.f * (4.f + K) * tt) * (1.f / 24.f)) +
+         q.dscale * eqref[4];
+  } else if constexpr (base == 1) {
+    fe = q.rho * (pf1_uv(a, b, t) + dt * (-6.f * tt + 4.f) * (1.f / 6.f)) + q.drho * eqref[1];
+    he = q.scale * (ph1_uv(K, t, a, b) + dt * (-2.f * c3 * t2s + (16.f + 4.f * K) * tt) * (1.f / 12.f)) +
+         q.dscale * eqref[5];
+  } else if constexpr (base == 5) {
+    fe = q.rho * (pf5_uv(a, b, t) + 0.25f * dt * tt) + q.drho * eqref[2];
+    he = q.scale * (ph5_uv(K, t, a, b) + dt * c3 * t2s * (1.f / 24.f)) + q.dscale * eqref[6];
+  } else {
+    fe = q.rho * (pf9_uv(a, t) + dt * (3.f * tt - 1.f) * (1.f / 24.f)) + q.drho * eqref[3];
+    he = q.scale * (ph9_uv(K, t, a, b) + dt * (c3 * t2s - (4.f + K) * tt) * (1.f / 48.f)) + q.dscale * eqref[7];
+  }
+}
+
 // Moments of one cell (reference macroProp, cpuLBM.cpp:916-970), FP64.
 // The relaxation times enter every update as divisors; the cell keeps their
 // reciprocals: 1/tf = p/mu, 1/th = Pr/tf, and since thf = th tf/(tf - th),
@@ -174,6 +265,23 @@ __device__ __forceinline__ void finish_macro(const InstDev& in, double rho, doub
   m.itf = in.visc_model == 0 ? m.p * in.imu : m.p / viscosity_of(in, m.T);
   m.ith = m.itf * in.Pr;
   m.ithf = m.itf * in.Pr_m1;
+}
+
+__device__ __forceinline__ EqDev eq_dev_inputs(const InstDev& in, const Macro& m) {
+  EqDev q;
+  const double t = m.T * in.iTc;
+  q.u = float(m.ux * in.isRTc);
+  q.v = float(m.uy * in.isRTc);
+  q.t = float(t);
+  q.dt = float(t - in.t0);
+  q.tt = float(t + in.t0);
+  q.t2s = float(t * t + t * in.t0 + in.t0 * in.t0);
+  q.rho = float(m.rho);
+  q.drho = float(m.rho - in.rho0);
+  q.K = float(in.K);
+  q.scale = float(m.rho * in.RTc);
+  q.dscale = float((m.rho - in.rho0) * in.RTc);
+  return q;
 }
 
 template <typename T>

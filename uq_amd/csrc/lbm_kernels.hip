@@ -223,20 +223,34 @@ __device__ __forceinline__ double hval(const CellState<TS>& c) {
   }
 }
 
-// Equilibrium of direction I in the stored form (FP32 storage: minus the
-// reference); with SUM, the full f value joins the failure-check sums.
-template <int I, typename TS, bool SUM>
-__device__ __forceinline__ void eq_stored(const InstDev& in, const EqIn<double>& q, double& fe, double& he,
+// Equilibrium of direction I in the stored form: FP64 storage the closed
+// form in FP64; FP32 storage its deviation from the reference equilibrium,
+// evaluated in FP32 in the deviation form (eq_dev). With SUM, the value
+// joins the failure-check sums (FP32 storage: sums of deviations, to which
+// failure_mass adds the reference's mass).
+template <int I, typename TS, bool SUM, typename TC>
+__device__ __forceinline__ void eq_stored(const InstDev& in, const EqIn<TC>& q, const EqDev& qd, TC& fe, TC& he,
                                           double& sumfe, double& sumabs) {
-  eq_dir<I>(q, fe, he);
-  if constexpr (SUM) {
-    sumfe += fe;
-    sumabs += fabs(fe);
-  }
   if constexpr (sizeof(TS) == 4) {
-    fe -= in.ref[I];
-    he -= in.ref[kQ + I];
+    float a, b;
+    eq_dev<I>(qd, in.eqref, a, b);
+    fe = TC(a);
+    he = TC(b);
+  } else {
+    (void)qd;
+    eq_dir<I>(q, fe, he);
   }
+  if constexpr (SUM) {
+    sumfe += double(fe);
+    sumabs += fabs(double(fe));
+  }
+}
+
+// Sum of the equilibrium f of a cell from the failure-check sums.
+template <typename TS>
+__device__ __forceinline__ double failure_mass(const InstDev& in, double sumfe) {
+  if constexpr (sizeof(TS) == 4) return sumfe + in.refsum[0];
+  else return sumfe;
 }
 
 // Stored value of a full distribution value computed in FP64: the
@@ -318,12 +332,13 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const
   Macro m;
   finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
   const Rec12<TC> r = make_rec12<TC>(in, m);
+  const EqDev qd = eq_dev_inputs(in, m);
   const TC hdt = TC(in.dt) / TC(2);
   double sumfe = 0, sumabs = 0;
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC fe, he, f2, h2;
-    eq_stored<I, TS, true>(in, r.q, fe, he, sumfe, sumabs);
+    eq_stored<I, TS, true>(in, r.q, qd, fe, he, sumfe, sumabs);
     s12<I>(r, hdt, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f2, h2);
     // streaming stores: f2 and, below, f3 and G are written once per step
     // and read back only by the next kernel, long after L2 has turned over
@@ -331,7 +346,7 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const
     __builtin_nontemporal_store(TS(h2), &dst[P * (kQ + I) + o]);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
+  flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
 }
 
 template <typename TS, typename TC>
@@ -396,6 +411,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     (void)mom_out;
   }
   const Rec12<TC> r = make_rec12<TC>(in, m1);
+  const EqDev qd1 = eq_dev_inputs(in, m1), qd3 = eq_dev_inputs(in, m3);
   const TC hdt = TC(in.dt) * TC(0.5);
   const EqIn<TC> q3 = eq_inputs<TC>(in, m3);
   const TC af3 = hdt * TC(m3.itf), ah3 = hdt * TC(m3.ith), ithf3 = TC(m3.ithf);
@@ -411,7 +427,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     }
     const TC fv = TC(fval<I>(cs)), hv = TC(hval<I>(cs));
     TC fe, he, f2, h2;
-    eq_stored<I, TS, false>(in, r.q, fe, he, sumfe, sumabs);
+    eq_stored<I, TS, false>(in, r.q, qd1, fe, he, sumfe, sumabs);
     s12<I>(r, hdt, fv, hv, fe, he, f2, h2);
     const TC D = fe - f2;
     const TC eu2 = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
@@ -419,7 +435,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     const TC Pf = (fv - dF) + r.af * D;
     const TC Qh = ((hv - dH) - hdt * A2) + r.ah * (he - h2);
     TC fe3, he3;
-    eq_stored<I, TS, true>(in, q3, fe3, he3, sumfe, sumabs);
+    eq_stored<I, TS, true>(in, q3, qd3, fe3, he3, sumfe, sumabs);
     const TC f3 = (Pf + af3 * fe3) * iaf3;
     const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
     const TC A3 = (eu3 * ithf3) * (fe3 - f3);
@@ -435,7 +451,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     if constexpr (sizeof(TS) == 4) __builtin_amdgcn_sched_barrier(0);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m3.rho, sumfe, eq_check_tol<TC>(sumabs));
+  flag_failure(in, step, m3.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
 }
 
 // Frozen cell (permanent or solid) of a pointwise stage: the stored values
@@ -459,7 +475,7 @@ __device__ __forceinline__ void copy_frozen(const InstDev& in, const TS* src, TS
 // it at 256 costs a few spills and hides twice the memory latency (4096^2
 // FP64: 5.9 -> 4.2 ms per launch).
 template <typename TS, typename TC>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(sizeof(TC) == 4 ? 4 : 2))) void k_s3(
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k_s3(
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
@@ -531,17 +547,18 @@ __device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, const 
   Macro m;
   finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
   const Rec1<TC> r = make_rec1<TC>(in, m);
+  const EqDev qd = eq_dev_inputs(in, m);
   double sumfe = 0, sumabs = 0;
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC fe, he, f1, h1;
-    eq_stored<I, TS, true>(in, r.q, fe, he, sumfe, sumabs);
+    eq_stored<I, TS, true>(in, r.q, qd, fe, he, sumfe, sumabs);
     s1<I>(r, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f1, h1);
     dst[P * I + o] = TS(f1);
     dst[P * (kQ + I) + o] = TS(h1);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
+  flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
 }
 
 template <typename TS, typename TC>
@@ -577,6 +594,7 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
   Macro m;
   finish_macro(in, cs.s[0], cs.s[1], cs.s[2], cs.s[3], m);
   const Rec1<TC> r = make_rec1<TC>(in, m);
+  const EqDev qd = eq_dev_inputs(in, m);
   double sumfe = 0, sumabs = 0;
   double t[4] = {0, 0, 0, 0};  // conserved sums of the new state (FP32 storage)
   auto one = [&](auto Ic) {
@@ -591,13 +609,13 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
     TC nf, nh;
     if constexpr (METHOD == 0) {
       TC fe, he, f1, h1;
-      eq_stored<I, TS, false>(in, r.q, fe, he, sumfe, sumabs);
+      eq_stored<I, TS, false>(in, r.q, qd, fe, he, sumfe, sumabs);
       s1<I>(r, fv, hv, fe, he, f1, h1);
       nf = fv - dF + r.bf * (fe - f1);
       nh = hv - dH + r.bh * (he - h1) - ((r.dt * eu) * r.ithf) * (fe - f1);
     } else {
       TC fe, he;
-      eq_stored<I, TS, true>(in, r.q, fe, he, sumfe, sumabs);
+      eq_stored<I, TS, true>(in, r.q, qd, fe, he, sumfe, sumabs);
       if constexpr (METHOD == 2) {
         nf = fv - dF + r.bf * (fe - fv);
         nh = hv - dH + r.bh * (he - hv) - ((r.dt * eu) * r.ithf) * (fe - fv);
@@ -622,7 +640,7 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
   } else {
     (void)mom_out;
   }
-  if (METHOD != 0) flag_failure(in, step, m.rho, sumfe, eq_check_tol<TC>(sumabs));
+  if (METHOD != 0) flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
 }
 
 template <typename TS, typename TC, int METHOD>
@@ -1682,9 +1700,14 @@ static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 // Persistent grid sizes of every step kernel of one storage/compute type and
 // flux tile shape, computed once (before any stream capture).
 // Compute type of the pointwise stage kernels and the wall closure: FP64
-// for both storage types (the equilibria and stage updates of FP32-stored
-// deviations are formed in FP64 and rounded once, on the way out).
+// for both storage types. FP32 storage evaluates the equilibria in FP32 as
+// deviations from the reference equilibrium (eq_dev), whose every term is
+// of the deviation's size, and forms the stage updates in FP64 and rounds
+// them once, on the store: stage updates in FP32 (five or more roundings of
+// the deviation per stored value) triple the error over 5000 steps.
+template <typename TS>
 using TP = double;
+using TW = double;
 
 // The stage-3 flux kernel also sums the moments of dF(f3) for the FP64
 // moment planes of FP32 storage.
@@ -1695,16 +1718,16 @@ template <typename TS, typename TC, typename Sh>
 struct StepGrids {
   int s12, flux2, s3, flux3, s1, flux0, fluxe, fin0, fin2, fin3;
   static const StepGrids& get() {
-    static const StepGrids g = {grid_for((const void*)k_s12<TS, TP>, kNT),
+    static const StepGrids g = {grid_for((const void*)k_s12<TS, TP<TS>>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s3<TS, TP>, kNT),
+                                grid_for((const void*)k_s3<TS, TP<TS>>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s1<TS, TP>, kNT),
+                                grid_for((const void*)k_s1<TS, TP<TS>>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT),
                                 grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_fin<TS, TP, 0>, kNT),
-                                grid_for((const void*)k_fin<TS, TP, 2>, kNT),
-                                grid_for((const void*)k_fin<TS, TP, 3>, kNT)};
+                                grid_for((const void*)k_fin<TS, TP<TS>, 0>, kNT),
+                                grid_for((const void*)k_fin<TS, TP<TS>, 2>, kNT),
+                                grid_for((const void*)k_fin<TS, TP<TS>, 3>, kNT)};
     return g;
   }
 };
@@ -1731,22 +1754,22 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
-  k_s12<TS, TP><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_s12<TS, TP<TS>><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(0, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TP><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, par, step);
+    k_alpha<TS, TW><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 0, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
   FluxKernel<TS, TC, 0, true, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_s3<TS, TP><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_s3<TS, TP<TS>><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TP><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, par, step);
+    k_alpha<TS, TW><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, 1, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(4, s);
@@ -1766,19 +1789,19 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
   const int wb = (L.nwalls + 255) / 256;
   if constexpr (METHOD == 0) {
     if (tm) tm->begin(0, s);
-    k_s1<TS, TP><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+    k_s1<TS, TP<TS>><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
     if (tm) tm->end(0, s);
   }
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
-    k_alpha<TS, TP><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, METHOD == 0 ? 0 : 2, par, step);
+    k_alpha<TS, TW><<<wb, 256, 0, s>>>(L.insts, L.walls, L.nwalls, METHOD == 0 ? 0 : 2, par, step);
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
   FluxKernel<TS, TC, FM, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_fin<TS, TP, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_fin<TS, TP<TS>, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
 }
 

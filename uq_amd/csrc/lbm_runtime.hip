@@ -336,6 +336,8 @@ static void set_reference(Solver& s) {
     s.ref[i] = f[i];
     s.ref[kQ + i] = h[i];
   }
+  s.ref_rho = rho;
+  s.ref_t = q.t;
 }
 
 Solver::~Solver() {
@@ -381,6 +383,16 @@ static void fill_inst(const Solver& s, InstDev& d) {
       d.refsum[3] += s.ref[kQ + i];
     }
   d.i2c = 0.5 / s.c;
+  if (s.precision != LBM_PREC_FP64) {
+    // base polynomials of the reference: f per unit density, h per unit rho R Tc
+    static constexpr int kBase[4] = {0, 1, 5, 9};
+    d.rho0 = s.ref_rho;
+    d.t0 = s.ref_t;
+    for (int j = 0; j < 4; j++) {
+      d.eqref[j] = float(s.ref[kBase[j]] / s.ref_rho);
+      d.eqref[4 + j] = float(s.ref[kQ + kBase[j]] / (s.ref_rho * s.p.R * s.Tc));
+    }
+  }
   d.dt = s.dt;
   d.dx = s.p.dx;
   d.dy = s.p.dy;

@@ -60,6 +60,7 @@ struct Solver {
   void* dflux = nullptr;
   double* mom[2] = {nullptr, nullptr};  // FP32 storage: conserved moments of state[k] (4 planes)
   double ref[kPlanes] = {};             // FP32 storage: reference equilibrium (deviation origin)
+  double ref_rho = 0, ref_t = 0;        // its density and T/Tc
   uint8_t* ctype = nullptr;
   double* alpha = nullptr;
   int* fail = nullptr;
