@@ -31,7 +31,9 @@ typedef enum lbm_status {
   LBM_FAILED = 1,          /* equilibrium mass check or non-finite state */
   LBM_ERR_ARG = -1,        /* bad argument */
   LBM_ERR_HIP = -2,        /* HIP runtime error */
-  LBM_ERR_GEOMETRY = -3,   /* geometry the fused tile kernels do not support */
+  LBM_ERR_GEOMETRY = -3,   /* reserved: no geometry the reference accepts is refused
+                              (thin solids and solids near mirror edges take
+                              the physical-addressing path) */
   LBM_ERR_NOMEM = -4,      /* device allocation failed or budget exceeded */
   LBM_ERR_STATE = -5,      /* call out of order (e.g. step before init) */
   LBM_ERR_NOGPU = -6       /* no usable gfx950 device */
@@ -184,11 +186,14 @@ int lbm_set_viscosity(lbm_solver* s, int32_t model, double mu_ref, double T_ref,
  * dy[ny] per row. Face fluxes are weighted by their face length and the net
  * flux divided by the cell area, i.e. the cell's own dt/dx[x] and dt/dy[y]
  * in cFLUX; dt is taken from the smallest spacing
- * (CODE/Python/simLB/LBM.py:216) and the step count re-derived. Before the
- * first step (LBM_ERR_STATE after); LBM_ERR_ARG for non-positive spacings
- * and for decks with solid cells (the OpenCL wall closure is not the C++
- * one this library follows). Totals (lbm_get_totals) stay unweighted cell
- * sums; the conserved quantities are the sums weighted by dx[x] dy[y]. */
+ * (CODE/Python/simLB/LBM.py:216) and the step count re-derived. Walls keep
+ * the C++ closure (CODE/C++/VS2010/simLB/simLB/cpuLBM.cpp:1502-1590) with
+ * the front end's face-length weights, which reduces to the wall cell's
+ * own dx, dy in its mass balance (the OpenCL closure's equilibrium-based
+ * outgoing flux, codeCL.py:685-753, is not adopted). Before the first step
+ * (LBM_ERR_STATE after); LBM_ERR_ARG for non-positive spacings. Totals
+ * (lbm_get_totals) stay unweighted cell sums; the conserved quantities are
+ * the sums weighted by dx[x] dy[y]. */
 int lbm_set_spacing(lbm_solver* s, const double* dx, const double* dy);
 
 /* Conserved totals over the fluid cells (cellType 0), FP64 sums:

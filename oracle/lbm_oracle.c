@@ -435,12 +435,24 @@ static void fluid_fluxes(orc* o, const double* f, const double* h, double* ff, d
     }
 }
 
-/* wallFluxes: mass-balanced diffuse wall, CPP/cpuLBM.cpp:1502-1590 */
+/* wallFluxes: mass-balanced diffuse wall, CPP/cpuLBM.cpp:1502-1590. On a
+ * non-uniform grid the fluxes through the wall cell's faces are weighted by
+ * their face lengths as the OpenCL front end weights every face flux
+ * (dy for x faces, dx for y faces, PY/codeCL.py:676-679, 722-737); a wall
+ * cell's axis neighbours share its row (x faces) or column (y faces), so the
+ * balance is the C++ one with the wall cell's own dx[x] and dy[y], which is
+ * the deck's dx, dy on a uniform grid. */
 static void wall_fluxes(orc* o, const double* f, double* ff, const double* h, double* hf) {
-  const double dx = o->p.dx, dy = o->p.dy;
   for (int x = 0; x < o->nx; x++)
     for (int y = 0; y < o->ny; y++) {
       if (cell_type_at(o, x, y) != 2) continue;
+#if defined(ORC_PLANT_SPACING) && ORC_PLANT_SPACING == 3
+      /* planted mistake, built only by the pin test: the deck's uniform
+       * spacing in the wall balance on a non-uniform grid */
+      const double dx = o->p.dx, dy = o->p.dy;
+#else
+      const double dx = o->dxs[x], dy = o->dys[y];
+#endif
       double in_f = 0, out_f = 0, in_h = 0, out_h = 0;
       for (int j = 0; j < 4; j++) {
         int mir_x, mir_y;
@@ -1008,18 +1020,15 @@ int orc_set_viscosity(orc* o, int32_t model, double mu_ref, double T_ref, double
  * their face length (dy[y] for x faces, dx[x] for y faces) and the net
  * divided by the cell area (PY/codeCL.py:611-618, 676-679). The two cells
  * of a face share its length, so cFLUX becomes dt/dx[x] and dt/dy[y] of the
- * cell; dt comes from the smallest spacing (PY/LBM.py:216). Refused (-1) for
- * non-positive spacings and for decks with solid cells, whose OpenCL wall
- * closure differs from the C++ one this oracle follows. */
+ * cell; dt comes from the smallest spacing (PY/LBM.py:216). Walls keep the
+ * C++ closure with face-length weights (wall_fluxes). Refused (-1) for
+ * non-positive spacings. */
 int orc_set_spacing(orc* o, const double* dx, const double* dy) {
   if (!o || !dx || !dy) return -1;
   for (int x = 0; x < o->nx; x++)
     if (!(dx[x] > 0) || !isfinite(dx[x])) return -1;
   for (int y = 0; y < o->ny; y++)
     if (!(dy[y] > 0) || !isfinite(dy[y])) return -1;
-  for (int x = 0; x < o->nx; x++)
-    for (int y = 0; y < o->ny; y++)
-      if (cell_type_at(o, x, y) == 2) return -1;
   double mn = dx[0];
   for (int x = 0; x < o->nx; x++) {
     o->dxs[x] = dx[x];

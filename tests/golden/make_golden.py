@@ -59,8 +59,10 @@ FAILURES = {
     # the four-quadrant Riemann deck of the reference's Lax-Liu generator
     # (physical air viscosity, CFL 0.2 as makeLaxLiu3.py:22): stable over
     # 2000 steps at 64^2 and 96^2, it trips the equilibrium check at 128^2
-    # near step 1050 (and near there on finer grids too)
-    "riemann128_failure": ("riemann", [128], {}, 2000, 6),
+    # near step 1050 (and near there on finer grids too); one-ulp
+    # perturbations move the failure over roughly 1010-1150, so the spread
+    # is sampled with 30 seeds
+    "riemann128_failure": ("riemann", [128], {}, 2000, 30),
 }
 
 

@@ -3,9 +3,9 @@ and per-row dy, PY/codeCL.py:611-618, 676-679) on the GPU: uniform arrays
 reproduce the uniform path bit for bit; stretched grids match the oracle
 within max(1e-12, 8 x the oracle's one-ulp sensitivity) relative L2 (a
 doubly periodic deck whose state varies along both axes, and the clamped
-Riemann deck); a batch mixing spaced and uniform
-members gives each member's result alone; decks with solid cells and late
-calls are refused."""
+Riemann deck, and decks with walls: the C++ wall closure with the wall cell's
+own dx, dy); a batch mixing spaced and uniform members gives each member's
+result alone; non-positive spacings and late calls are refused."""
 import numpy as np
 import pytest
 
@@ -142,14 +142,40 @@ def test_mixed_precision_on_a_stretched_grid(gpu_lib):
     assert st64 == 0 and st32 == 0 and e <= 1e-6
 
 
-def test_refusals(gpu_lib):
-    wall = cases.couette(24, steps=5)
-    s = gpu_lib.Solver(wall, "fp64")
+@pytest.mark.parametrize("deck", ["obstacle", "couette"])
+def test_stretched_grid_with_walls_matches_oracle(gpu_lib, deck):
+    """Walls on a non-uniform grid: flow past a block (its corner cells
+    have fluid on both axes, where the wall balance's x and y weights do not
+    cancel) and the Couette channel, against the oracle; the area-weighted
+    fluid mass is conserved (the balance's purpose)."""
+    if deck == "obstacle":
+        base = cases.obstacle(40, block=(15, 15, 6, 6), u0=0.15, steps=30)
+    else:
+        base = cases.couette(40, u_wall=0.2, steps=30)
+    c = cases.stretched(base, amp=0.3)
+    st_o, fo, ho = _oracle(c, 30, True)
+    _, fp, hp = _oracle(c, 30, True, perturb=True)
+    bar = max(1e-12, 8.0 * max(rel_l2(fp, fo), rel_l2(hp, ho)))
+    s = gpu_lib.Solver(c, "fp64")
     try:
-        with pytest.raises(LbmError):
-            s.set_spacing(np.full(wall.nx, wall.params["dx"]), np.full(wall.ny, wall.params["dy"]))
+        _modulate(s.get_dist, s.set_dist, c.nx, c.ny)
+        f0, _ = s.get_dist()
+        st_g = s.step(30)
+        fg, hg = s.get_dist()
     finally:
         s.close()
+    ef, eh = rel_l2(fg, fo), rel_l2(hg, ho)
+    print(f"[spacing] {c.name} with walls: f {ef:.2e} h {eh:.2e} bar {bar:.2e}", flush=True)
+    assert st_o == 0 and st_g == 0
+    assert ef <= bar and eh <= bar
+    vol = np.outer(c.meta["spacing"]["dx"], c.meta["spacing"]["dy"])
+    fluid = np.asarray(c.bnd) == 0 if deck == "obstacle" else np.pad(np.ones((c.nx, c.ny - 2), bool), ((0, 0), (1, 1)))
+    m0, m1 = (vol * f0.sum(axis=2))[fluid].sum(), (vol * fg.sum(axis=2))[fluid].sum()
+    print(f"[spacing] {c.name}: weighted fluid mass drift {abs(m1 - m0) / m0:.1e}", flush=True)
+    assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_refusals(gpu_lib):
     c = cases.shear_wave(16, steps=5)
     s = gpu_lib.Solver(c, "fp64")
     try:

@@ -522,11 +522,79 @@ def test_spacing_time_step_and_refusals():
     with pytest.raises(ValueError):
         o.set_spacing(dx, -dy)
     o.close()
-    wall = cases.couette(16, steps=5)                                  # solid rows: refused
-    ow = Oracle(wall)
-    with pytest.raises(ValueError):
-        ow.set_spacing(np.full(wall.nx, wall.params["dx"]), np.full(wall.ny, wall.params["dy"]))
-    ow.close()
+
+
+# --------------------------------------------------------------------------
+# Spacing with walls: the C++ wall closure (cpuLBM.cpp:1502-1590) with the
+# OpenCL front end's face-length weights (PY/codeCL.py:676-679, 722-737),
+# i.e. the wall cell's own dx, dy in its mass balance. Pinned by what the
+# balance is for: with walls (and no permanent cells) the fluid's
+# area-weighted mass sum rho dx dy changes only by the wall faces' net
+# flux, which the balance makes zero, so it is conserved to rounding on a
+# stretched grid; and with uniform arrays the scalar path is reproduced
+# bit for bit.
+# --------------------------------------------------------------------------
+def _walls_weighted_mass(o, dx, dy, fluid):
+    f, _ = o.get_dist()
+    return float((np.outer(dx, dy) * f.sum(axis=2))[fluid].sum())
+
+
+def _stretched_obstacle(rk=1, fm=1):
+    """Flow past a 6 x 6 block on a stretched doubly periodic grid: the
+    block's corner cells have fluid neighbours on both axes, where the
+    wall balance's x and y weights do not cancel."""
+    base = cases.obstacle(32, block=(12, 12, 6, 6), u0=0.15, steps=60, rk_method=rk, flux_method=fm)
+    c = cases.stretched(base, amp=0.3)
+    fluid = np.asarray(c.bnd) == 0
+    return c, np.array(c.meta["spacing"]["dx"]), np.array(c.meta["spacing"]["dy"]), fluid
+
+
+@pytest.mark.parametrize("rk,fm", [(1, 1), (0, 1), (1, 0)], ids=["rk3", "rk1", "rk3_nnd"])
+def test_spacing_with_walls_conserves_area_weighted_mass(rk, fm):
+    c, dx, dy, fluid = _stretched_obstacle(rk, fm)
+    o = Oracle(c)
+    _modulate(o, c.nx, c.ny)
+    m0 = _walls_weighted_mass(o, dx, dy, fluid)
+    assert o.step(60) == 0
+    m1 = _walls_weighted_mass(o, dx, dy, fluid)
+    f_spaced, _ = o.get_dist()
+    o.close()
+    print(f"stretched obstacle rk{rk} fm{fm}: weighted mass drift {abs(m1 - m0) / m0:.1e}")
+    assert abs(m1 - m0) <= 1e-13 * m0
+    # the spacing matters: the uniform deck ends elsewhere
+    u = Oracle(cases.obstacle(32, block=(12, 12, 6, 6), u0=0.15, steps=60, rk_method=rk, flux_method=fm))
+    _modulate(u, c.nx, c.ny)
+    u.step(60)
+    f_uniform, _ = u.get_dist()
+    u.close()
+    assert np.abs(f_spaced - f_uniform).max() > 1e-6
+
+
+def test_spacing_with_walls_pin_catches_uniform_wall_balance(tmp_path):
+    """An oracle whose wall balance keeps the deck's uniform dx, dy on the
+    stretched grid leaks mass through the walls."""
+    mod = _planted_oracle(tmp_path, 3)
+    c, dx, dy, fluid = _stretched_obstacle()
+    o = mod.Oracle(c)
+    _modulate(o, c.nx, c.ny)
+    m0 = _walls_weighted_mass(o, dx, dy, fluid)
+    o.step(60)
+    m1 = _walls_weighted_mass(o, dx, dy, fluid)
+    o.close()
+    assert abs(m1 - m0) > 1e-9 * m0, abs(m1 - m0) / m0
+
+
+def test_spacing_with_walls_uniform_arrays_are_the_scalar_path():
+    c = cases.obstacle(24, block=(8, 8, 5, 5), u0=0.15, steps=10)
+    a = Oracle(c)
+    a.step(10)
+    fa, ha = a.get_dist()
+    a.close()
+    b = Oracle(cases.with_spacing(c, np.full(c.nx, c.params["dx"]), np.full(c.ny, c.params["dy"])))
+    b.step(10)
+    fb, hb = b.get_dist()
+    b.close()
+    assert np.array_equal(fa, fb) and np.array_equal(ha, hb)
 
 
 # --------------------------------------------------------------------------

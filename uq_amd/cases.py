@@ -131,6 +131,36 @@ def shear_wave(n: int = 64, u0: float = 0.1, T: float = 1.0, rho: float = 1.0,
     return Case(f"shear{n}", params, bnd, _props(rows), steps, meta)
 
 
+def obstacle(n: int = 48, block=(18, 20, 8, 6), u0: float = 0.1, T: float = 1.0, rho: float = 1.0,
+             tau: float = 0.05, Pr: float = 0.71, gamma: float = 1.4, R: float = 1.0,
+             cfl: float = 0.3, steps: int = 200, flux_method: int = 1, rk_method: int = 1,
+             length: float = 1.0, mirror_south: bool = False) -> Case:
+    """Uniform flow u0 along x in an n x n box, periodic in x and in y (with
+    `mirror_south`: mirrored at the south edge, clamped at the north edge),
+    around a solid block (isothermal, at rest) of w x h cells with its
+    lower-left cell at (x0, y0), block = (x0, y0, w, h).
+    The block's corner cells are wall cells with fluid neighbours on both
+    axes; h = 1 or w = 1 gives a one-cell-thick plate, and y0 = 0 with
+    `mirror_south` a solid on a mirror edge. dx = dy = length/n."""
+    x0, y0, w, h = block
+    nx = ny = n
+    p = rho * R * T
+    bnd = np.zeros((nx, ny), dtype=np.int32)
+    bnd[x0:x0 + w, y0:y0 + h] = 1
+    props = _props([
+        (rho, p, T, u0, 0.0, FLUID),
+        (rho, p, T, 0.0, 0.0, SOLID),
+    ])
+    d = length / n
+    params = _params(rk_method=rk_method, flux_method=flux_method, cfl=cfl,
+                     steps=steps, periodic_x=1, periodic_y=0 if mirror_south else 1,
+                     mirror_south=1 if mirror_south else 0, gamma=gamma, R=R, mu=tau * p,
+                     Pr=Pr, dx=d, dy=d, rho_ref=rho, p_ref=p, T_ref=T, Tc_Tref=1.0)
+    meta = dict(kind="obstacle", u_wall=u0, T=T, rho=rho, tau=tau, Pr=Pr, gamma=gamma, R=R,
+                length=length, block=list(block))
+    return Case(f"obstacle{n}_{x0}_{y0}_{w}x{h}", params, bnd, props, steps, meta)
+
+
 def flat_plate(nx: int = 512, ny: int = 256, mach: float = 2.0, re_x: float = 1e5,
                plate_start: int = 64, steps: int = 20000, flux_method: int = 0,
                rk_method: int = 1, cfl: float = 0.3, wall: str = "recovery") -> Case:

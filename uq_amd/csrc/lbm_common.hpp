@@ -75,10 +75,13 @@ struct InstDev {
   // mu_ref (T/T_ref)^1.5 (T_ref + vS)/(T + vS)
   int visc_model;
   double mu_ref, T_ref_v, iT_ref_v, vexp, vS;
-  // non-uniform grid: dt/dx per column [nx] and dt/dy per row [ny]; null
-  // for a uniform grid (rdx, rdy above)
+  // non-uniform grid: dt/dx per column [nx] and dt/dy per row [ny], and
+  // dx per column and dy per row (the wall balance); null for a uniform grid
+  // (rdx, rdy, dx, dy above)
   const double* rdxs;
   const double* rdys;
+  const double* dxs;
+  const double* dys;
   // residual tracking of a served request with a tolerance (the reference
   // driver's stop rule, main.cpp:54-110): res_hist[k - 2] receives the
   // residual of global step k for 2 <= k <= 20, res_hist[19] that of global
@@ -96,10 +99,11 @@ constexpr int kResHist = 20;  // residual history slots per tracked instance
 struct TileRef {
   int inst;
   int tx, ty;
-  int flags;           // kTileSolid: the tile's halo region holds a solid cell
+  int flags;           // kTileSolid: the tile's halo region holds a solid cell;
+                       // kTilePhysical: an own cell needs the reference's physical addressing
 };
 
-enum : int { kTileSolid = 1 };
+enum : int { kTileSolid = 1, kTilePhysical = 2 };
 
 // Wall cell that needs a closure factor (has a non-solid axis neighbour).
 struct WallRef {

@@ -1020,6 +1020,144 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
 // so their latency hides behind direction I's net fluxes.
 // ---------------------------------------------------------------------------
 
+
+// ---------------------------------------------------------------------------
+// The reference's physical stencil addressing (cpuLBM.cpp:1171-1326,
+// 1436-1470, 1502-1590), for the wall closure and for the tiles whose cells
+// the tile kernels' logical addressing would get wrong (next to solids one
+// cell thick or near a mirror edge): there each own cell's net flux is
+// formed from global memory, face by face, as the reference forms it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ref_index(const InstDev& in, int i, int pm, int d, int& mir) {
+  unsigned m, cl;
+  int r;
+  if (d == 0)
+    r = map_axis(i + pm, in.nx, in.flags & kPeriodicX, in.flags & kMirrorW, in.flags & kMirrorE, m, cl);
+  else
+    r = map_axis(i + pm, in.ny, in.flags & kPeriodicY, in.flags & kMirrorS, in.flags & kMirrorN, m, cl);
+  mir = (int)m;
+  return r;
+}
+
+// Value of plane k at offset o in FP64 (FP32 storage: deviation + reference).
+template <typename TS>
+__device__ __forceinline__ double full_val(const InstDev& in, const TS* src, int k, long long o) {
+  if constexpr (sizeof(TS) == 4) return double(src[in.plane * k + o]) + in.ref[k];
+  else return double(src[in.plane * k + o]);
+}
+
+// Face flux of distribution k0 + i (k0 = 0: f, kQ: h) out of fluid cell
+// (x, y) along `axis`, with the reference's physical addressing: the
+// stencil line through the mapped neighbours (direction remapped at a
+// mirror), its first solid run extrapolated from the two nearest fluid
+// points of the line in direction i; val(k, o) gives the FP64 value of
+// plane k at offset o.
+template <typename TC, typename Val>
+__device__ TC ref_line_flux(const InstDev& in, const Val& val, int k0, int x, int y, int i, int axis) {
+  const int e = axis == 0 ? c_lat_x[i] : c_lat_y[i];
+  const int s = e > 0 ? 1 : (e < 0 ? -1 : 0);
+  if (s == 0) return TC(0);
+  const int L = in.flux_method == 1 ? 5 : 3;
+  const int low = (L - 1) / 2;
+  const int start = (axis == 0 ? x : y) - s * low;
+  TC V[5];
+  int sld[5], any = 0;
+  for (int j = 0; j < L; j++) {
+    int mir;
+    const int p = ref_index(in, start, s * j, axis, mir);
+    const int d = mir ? (axis == 0 ? c_mir_ew[i] : c_mir_ns[i]) : i;
+    const long long o = axis == 0 ? (long long)y * in.pitch + p : (long long)p * in.pitch + x;
+    V[j] = TC(val(k0 + d, o));
+    sld[j] = in.ctype[o] == 2;
+    any |= sld[j];
+  }
+  if (any) {
+    int diff = 0, jj = 0, pos = 0, mir;
+    for (int j = 0; j < L - 1; j++) {
+      diff = sld[j] - sld[j + 1];
+      if (diff > 0) { jj = j; pos = ref_index(in, start, s * j, axis, mir); break; }
+      if (diff < 0) { jj = j + 1; pos = ref_index(in, start, s * (j + 1), axis, mir); break; }
+    }
+    const int m1 = ref_index(in, pos, s * diff, axis, mir);
+    const int m2 = ref_index(in, pos, s * 2 * diff, axis, mir);
+    const long long o1 = axis == 0 ? (long long)y * in.pitch + m1 : (long long)m1 * in.pitch + x;
+    const long long o2 = axis == 0 ? (long long)y * in.pitch + m2 : (long long)m2 * in.pitch + x;
+    V[jj] = TC(2) * TC(val(k0 + i, o1)) - TC(val(k0 + i, o2));
+    const int nx = jj - diff;
+    if (nx >= 0 && nx < L) V[nx] = V[jj];
+  }
+  const TC v = TC(double(e > 0 ? e : -e) * in.c);
+  if (L == 5) return weno5(v, V[0], V[1], V[2], V[3], V[4]);
+  return nnd(v, V[0], V[1], V[2]);
+}
+
+// Face fluxes (f, h) of direction i out of cell (x, y) along `axis`: the
+// wall closure alpha |e| f at a wall owner, else the reference line.
+template <typename Val>
+__device__ void phys_face(const InstDev& in, const Val& val, int x, int y, int i, int axis, double& Ff,
+                          double& Fh) {
+  const long long o = (long long)y * in.pitch + x;
+  if (in.ctype[o] == 2) {
+    const int e = axis == 0 ? c_lat_x[i] : c_lat_y[i];
+    const double av = in.alpha[o] * (double(e > 0 ? e : -e) * in.c);
+    Ff = av * val(i, o);
+    Fh = av * val(kQ + i, o);
+    return;
+  }
+  Ff = ref_line_flux<double>(in, val, 0, x, y, i, axis);
+  Fh = ref_line_flux<double>(in, val, kQ, x, y, i, axis);
+}
+
+// Net flux dF, dH of direction I out of cell (x, y) with the reference's
+// addressing (cFLUX, cpuLBM.cpp:1436-1470): out-faces of the cell, in-faces
+// of the upwind owner mapped through the boundary rules (the mirrored
+// direction across a mirror edge; the cell itself across a clamped one).
+// FP64 from the full values; combined as dir_net combines.
+template <typename TS, typename TC>
+__device__ void phys_net(const InstDev& in, const TS* src, int x, int y, int I, double rdx, double rdy, TC& dF,
+                         TC& dH) {
+  const auto val = [&](int k, long long o) { return full_val(in, src, k, o); };
+  const int ex = c_lat_x[I], ey = c_lat_y[I];
+  double oxf = 0, oxh = 0, ixf = 0, ixh = 0, oyf = 0, oyh = 0, iyf = 0, iyh = 0;
+  if (ex) {
+    phys_face(in, val, x, y, I, 0, oxf, oxh);
+    unsigned m, cl;
+    const int xu = map_axis(x - (ex > 0 ? 1 : -1), in.nx, in.flags & kPeriodicX, in.flags & kMirrorW,
+                            in.flags & kMirrorE, m, cl);
+    if (cl) {
+      ixf = oxf;
+      ixh = oxh;
+    } else {
+      phys_face(in, val, xu, y, m ? c_mir_ew[I] : I, 0, ixf, ixh);
+    }
+  }
+  if (ey) {
+    phys_face(in, val, x, y, I, 1, oyf, oyh);
+    unsigned m, cl;
+    const int yu = map_axis(y - (ey > 0 ? 1 : -1), in.ny, in.flags & kPeriodicY, in.flags & kMirrorS,
+                            in.flags & kMirrorN, m, cl);
+    if (cl) {
+      iyf = oyf;
+      iyh = oyh;
+    } else {
+      phys_face(in, val, x, yu, m ? c_mir_ns[I] : I, 1, iyf, iyh);
+    }
+  }
+  double f, h;
+  if (ex && ey) {
+    f = (rdx * oxf + rdy * oyf) - (rdx * ixf + rdy * iyf);
+    h = (rdx * oxh + rdy * oyh) - (rdx * ixh + rdy * iyh);
+  } else if (ex) {
+    f = rdx * oxf - rdx * ixf;
+    h = rdx * oxh - rdx * ixh;
+  } else {
+    f = rdy * oyf - rdy * iyf;
+    h = rdy * oyh - rdy * iyh;
+  }
+  dF = TC(f);
+  dH = TC(h);
+}
+
 // Per-thread state of one tile pass. The instance fields the direction loop
 // needs are copied here once per tile: read through the instance table
 // inside the loop (after the loop's stores) they would be vector loads,
@@ -1268,10 +1406,72 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
+    if (tr.flags & kTilePhysical) continue;  // k_phys forms this tile's net fluxes
     __syncthreads();
     setup_region(R, in, tr.tx * Sh::TX, tr.ty * Sh::TY);
     __syncthreads();
     flux_tile<TS, TC, MODE, MOM>(S, VA, VB, R, in, tr, par);
+  }
+}
+
+
+// k_phys<MODE>: the flux kernel's work for the tiles flagged kTilePhysical,
+// whose net fluxes the logical addressing of the staged tiles would get
+// wrong: each thread takes an own fluid cell and forms its twelve net
+// fluxes with the reference's addressing (phys_net), then writes what the
+// flux kernel writes for the cell (MODE 0 and 2: the dF planes, with MOM
+// the moment sums; MODE 1: the new state G - dF/2, with MOM the moment
+// planes' update). Launched after the flux kernel, only for groups with
+// such tiles; the flux kernel skips them.
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+__global__ __launch_bounds__(256) void k_phys(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
+                                              const int* stepp) {
+  const int step = *stepp;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    if (!(tr.flags & kTilePhysical)) continue;
+    const InstDev& in = insts[tr.inst];
+    if (frozen_by_failure(in, step)) continue;
+    const long long P = in.plane;
+    const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
+    const TS* src = reinterpret_cast<const TS*>(MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[cur]));
+    TS* dst = reinterpret_cast<TS*>(MODE == 1 ? in.state[nxt] : in.dflux);
+    for (int c = threadIdx.x; c < Sh::Own; c += blockDim.x) {
+      const int x = tr.tx * Sh::TX + c % Sh::TX, y = tr.ty * Sh::TY + c / Sh::TX;
+      if (x >= in.nx || y >= in.ny) continue;
+      const long long o = (long long)y * in.pitch + x;
+      if (in.ctype[o] != 0) continue;
+      const double rdx = in.rdxs ? in.rdxs[x] : in.rdx, rdy = in.rdys ? in.rdys[y] : in.rdy;
+      double m[4] = {0, 0, 0, 0};
+      for (int I = 1; I < kQ; I++) {
+        TC dF, dH;
+        phys_net<TS, TC>(in, src, x, y, I, rdx, rdy, dF, dH);
+        const double ex = c_lat_x[I] * in.c, ey = c_lat_y[I] * in.c;
+        double vf, vh;
+        if (MODE != 1) {
+          dst[P * (I - 1) + o] = TS(dF);
+          dst[P * (12 + I - 1) + o] = TS(dH);
+          vf = double(TS(dF));
+          vh = double(TS(dH));
+        } else {
+          dst[P * I + o] = TS(TC(dst[P * I + o]) - TC(0.5) * dF);
+          dst[P * (kQ + I) + o] = TS(TC(dst[P * (kQ + I) + o]) - TC(0.5) * dH);
+          vf = double(dF);
+          vh = double(dH);
+        }
+        m[0] += vf;
+        m[1] += vf * ex;
+        m[2] += vf * ey;
+        m[3] += vh;
+      }
+      if (MOM) {
+        double* mo = MODE == 1 ? in.mom[nxt] : in.dmom;
+        for (int k = 0; k < 4; k++) {
+          if (MODE == 1) mo[k * P + o] -= 0.5 * m[k];
+          else mo[k * P + o] = m[k];
+        }
+      }
+    }
   }
 }
 
@@ -1325,63 +1525,6 @@ __constant__ int c_wall_dy[4] = {0, 0, -1, 1};
 __constant__ int c_wall_in[4][4] = {{1, 5, 8, 9}, {3, 6, 7, 11}, {2, 5, 6, 10}, {4, 7, 8, 12}};
 __constant__ int c_wall_out[4][4] = {{3, 6, 7, 11}, {1, 5, 8, 9}, {4, 8, 7, 12}, {2, 6, 5, 10}};
 
-__device__ __forceinline__ int ref_index(const InstDev& in, int i, int pm, int d, int& mir) {
-  unsigned m, cl;
-  int r;
-  if (d == 0)
-    r = map_axis(i + pm, in.nx, in.flags & kPeriodicX, in.flags & kMirrorW, in.flags & kMirrorE, m, cl);
-  else
-    r = map_axis(i + pm, in.ny, in.flags & kPeriodicY, in.flags & kMirrorS, in.flags & kMirrorN, m, cl);
-  mir = (int)m;
-  return r;
-}
-
-// Value of plane k at offset o in FP64 (FP32 storage: deviation + reference).
-template <typename TS>
-__device__ __forceinline__ double full_val(const InstDev& in, const TS* src, int k, long long o) {
-  if constexpr (sizeof(TS) == 4) return double(src[in.plane * k + o]) + in.ref[k];
-  else return double(src[in.plane * k + o]);
-}
-
-template <typename TS, typename TC>
-__device__ TC ref_face_flux_f(const InstDev& in, const TS* src, int x, int y, int i, int axis) {
-  const int e = axis == 0 ? c_lat_x[i] : c_lat_y[i];
-  const int s = e > 0 ? 1 : (e < 0 ? -1 : 0);
-  if (s == 0) return TC(0);
-  const int L = in.flux_method == 1 ? 5 : 3;
-  const int low = (L - 1) / 2;
-  const int start = (axis == 0 ? x : y) - s * low;
-  TC V[5];
-  int sld[5], any = 0;
-  for (int j = 0; j < L; j++) {
-    int mir;
-    const int p = ref_index(in, start, s * j, axis, mir);
-    const int d = mir ? (axis == 0 ? c_mir_ew[i] : c_mir_ns[i]) : i;
-    const long long o = axis == 0 ? (long long)y * in.pitch + p : (long long)p * in.pitch + x;
-    V[j] = TC(full_val(in, src, d, o));
-    sld[j] = in.ctype[o] == 2;
-    any |= sld[j];
-  }
-  if (any) {
-    int diff = 0, jj = 0, pos = 0, mir;
-    for (int j = 0; j < L - 1; j++) {
-      diff = sld[j] - sld[j + 1];
-      if (diff > 0) { jj = j; pos = ref_index(in, start, s * j, axis, mir); break; }
-      if (diff < 0) { jj = j + 1; pos = ref_index(in, start, s * (j + 1), axis, mir); break; }
-    }
-    const int m1 = ref_index(in, pos, s * diff, axis, mir);
-    const int m2 = ref_index(in, pos, s * 2 * diff, axis, mir);
-    const long long o1 = axis == 0 ? (long long)y * in.pitch + m1 : (long long)m1 * in.pitch + x;
-    const long long o2 = axis == 0 ? (long long)y * in.pitch + m2 : (long long)m2 * in.pitch + x;
-    V[jj] = TC(2) * TC(full_val(in, src, i, o1)) - TC(full_val(in, src, i, o2));
-    const int nx = jj - diff;
-    if (nx >= 0 && nx < L) V[nx] = V[jj];
-  }
-  const TC v = TC(double(e > 0 ? e : -e) * in.c);
-  if (L == 5) return weno5(v, V[0], V[1], V[2], V[3], V[4]);
-  return nnd(v, V[0], V[1], V[2]);
-}
-
 template <typename TS, typename TC>
 __global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, int which, int par,
                         const int* stepp) {
@@ -1393,6 +1536,7 @@ __global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, 
   if (frozen_by_failure(in, step)) return;
   const void* srcv = which == 0 ? in.f2 : (which == 1 ? in.f3 : in.state[in.parity0 ^ par]);
   const TS* src = reinterpret_cast<const TS*>(srcv);
+  const auto fv = [&](int k, long long o) { return full_val(in, src, k, o); };
   const long long ow = (long long)wr.y * in.pitch + wr.x;
   double in_f = 0, out_f = 0;
   for (int j = 0; j < 4; j++) {
@@ -1402,10 +1546,13 @@ __global__ void k_alpha(const InstDev* insts, const WallRef* walls, int nwalls, 
     const long long on = (long long)yn * in.pitch + xn;
     if (in.ctype[on] >= 2) continue;
     const int axis = j < 2 ? 0 : 1;
-    const double dd = axis == 0 ? in.dx : in.dy;
+    // the wall cell's own spacing: its axis neighbours share its row (x
+    // faces) or column (y faces), so the face-length weights of a
+    // non-uniform grid reduce to it
+    const double dd = axis == 0 ? (in.dxs ? in.dxs[wr.x] : in.dx) : (in.dys ? in.dys[wr.y] : in.dy);
     for (int k = 0; k < 4; k++) {
       const int i = c_wall_in[j][k], io = c_wall_out[j][k];
-      const double F = double(ref_face_flux_f<TS, TC>(in, src, xn, yn, i, axis));
+      const double F = double(ref_line_flux<TC>(in, fv, 0, xn, yn, i, axis));
       in_f += F / dd;
       const double vv = fabs(double(axis == 0 ? c_lat_x[io] : c_lat_y[io]) * in.c);
       const double fw = full_val(in, src, io, ow);
@@ -1746,6 +1893,15 @@ void prepare_launch(int fp64) {
   else prepare_shapes<float, float>();
 }
 
+// The physical-addressing net fluxes of the flagged tiles, after the flux
+// kernel of the same MODE.
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+static void launch_phys(const StepLaunch& L, hipStream_t s, int par) {
+  if (L.nphys > 0)
+    k_phys<TS, TC, MODE, MOM, Sh><<<L.ntiles < 1024 ? L.ntiles : 1024, 256, 0, s>>>(L.insts, L.tiles, L.ntiles, par,
+                                                                                  L.step_ctr);
+}
+
 template <typename TS, typename TC, typename Sh>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
@@ -1763,6 +1919,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
   }
   if (tm) tm->begin(2, s);
   FluxKernel<TS, TC, 0, true, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  launch_phys<TS, TC, 0, true, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   k_s3<TS, TP<TS>><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
@@ -1774,6 +1931,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
   }
   if (tm) tm->begin(4, s);
   FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn<<<blocks_for(n, g3), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  launch_phys<TS, TC, 1, kMom3<TS>, Sh>(L, s, par);
   if (tm) tm->end(4, s);
 }
 
@@ -1799,6 +1957,7 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
   }
   if (tm) tm->begin(2, s);
   FluxKernel<TS, TC, FM, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  launch_phys<TS, TC, FM, false, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   k_fin<TS, TP<TS>, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);

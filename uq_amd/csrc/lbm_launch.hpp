@@ -27,12 +27,21 @@ struct StepLaunch {
   double* res_partial = nullptr;    // per pointwise tile residual partials (nptiles doubles) when
                                     // some instance tracks its residual; null otherwise
   int ninst = 0;                    // instances in the table (for the residual tracking)
+  int nphys = 0;                    // tiles flagged kTilePhysical in `tiles` (k_phys runs when > 0)
 };
+
+// Flux tiles of a list flagged kTilePhysical.
+inline int count_physical(const std::vector<TileRef>& tiles) {
+  int n = 0;
+  for (const TileRef& t : tiles) n += (t.flags & kTilePhysical) ? 1 : 0;
+  return n;
+}
 
 inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
   return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
          a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
-         a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape && a.res_partial == b.res_partial && a.ninst == b.ninst;
+         a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape && a.res_partial == b.res_partial && a.ninst == b.ninst &&
+         a.nphys == b.nphys;
 }
 
 // Optional per-kernel timing with HIP events recorded on the launch stream

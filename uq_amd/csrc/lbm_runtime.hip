@@ -93,13 +93,15 @@ static void steps_from_dt(Solver& s) {
 }
 
 // ---------------------------------------------------------------------------
-// Geometry validation: the tile kernels resolve a stencil point from its
-// logical position (cell + offset mapped once through the boundary rules,
-// direction remapped at mirrors) and take an upwind face flux across a
-// clamped edge from the cell itself. The reference instead maps the
+// Stencil addressing per cell: the tile kernels resolve a stencil point from
+// its logical position (cell + offset mapped once through the boundary
+// rules, direction remapped at mirrors) and take an upwind face flux across
+// a clamped edge from the cell itself. The reference instead maps the
 // physical neighbour of a physical point (extrapolation points, upwind
-// owners). The two agree except next to thin solid features or solids at a
-// mirror edge; this check proves agreement cell by cell for one geometry.
+// owners). The two agree except next to solid features one cell thick or
+// solids near a mirror edge; this check compares them cell by cell and
+// marks the cells where they differ, whose tiles then form every net flux
+// with the reference's addressing (phys_net in the flux kernels).
 // ---------------------------------------------------------------------------
 struct Pt {
   int x, y, d;
@@ -228,8 +230,12 @@ static bool same(const LineReads& a, const LineReads& b) {
   return a.jj == b.jj && a.diff == b.diff && a.e1 == b.e1 && a.e2 == b.e2;
 }
 
-static int validate_geometry(const Solver& s) {
+// Marks in s.phys_cells ([y][x]) every fluid or permanent cell one of whose
+// faces the logical addressing would form differently from the reference.
+static void mark_physical_cells(Solver& s) {
   const int R = 5;  // reach of owner (1) + stencil (2) + extrapolation (2)
+  s.phys_cells.assign((size_t)s.nx * s.ny, 0);
+  s.any_phys = false;
   for (int axis = 0; axis < 2; axis++) {
     const int n = axis == 0 ? s.nx : s.ny, m = axis == 0 ? s.ny : s.nx;
     std::vector<char> near(n);
@@ -246,7 +252,8 @@ static int validate_geometry(const Solver& s) {
         if (!near[k]) continue;
         const int cx = axis == 0 ? k : line, cy = axis == 0 ? line : k;
         if (type_at(s, cx, cy) == 2) continue;
-        for (int i = 1; i < kQ; i++) {
+        bool differs = false;
+        for (int i = 1; i < kQ && !differs; i++) {
           const int e = axis == 0 ? lat_x(i) : lat_y(i);
           if (e == 0) continue;
           const int sg = sgn_i(e);
@@ -254,34 +261,31 @@ static int validate_geometry(const Solver& s) {
           // own out-face
           ref_line(s, cx, cy, i, axis, a);
           ker_line(s, cx, cy, i, axis, b);
-          if (!same(a, b))
-            return fail_with(LBM_ERR_GEOMETRY,
-                             "geometry not supported by the tile kernels: stencil of cell (%d,%d) "
-                             "direction %d axis %d differs from the reference addressing",
-                             cx, cy, i, axis);
-          // upwind owner (reference cFLUX: index(x,-s) with mirror remap)
+          if (!same(a, b)) {
+            differs = true;
+            break;
+          }
+          // upwind owner (reference cFLUX: index(x,-s) with mirror remap);
+          // across a clamped edge both take the cell's own out-face
           int mir;
           bool cl;
           const int up = ref_map(s, (axis == 0 ? cx : cy) - sg, axis, mir, cl);
           const int iu = mir ? (axis == 0 ? mirror_ew(i) : mirror_ns(i)) : i;
           if (cl) {
-            if (up != (axis == 0 ? cx : cy))
-              return fail_with(LBM_ERR_GEOMETRY, "clamped upwind owner of (%d,%d) is not the cell itself",
-                               cx, cy);
-            continue;  // the kernel uses the cell's own out-face
+            differs = up != (axis == 0 ? cx : cy);
+            continue;
           }
           ref_line(s, axis == 0 ? up : cx, axis == 0 ? cy : up, iu, axis, a);
           ker_line(s, axis == 0 ? cx - sg : cx, axis == 0 ? cy : cy - sg, i, axis, b);
-          if (!same(a, b))
-            return fail_with(LBM_ERR_GEOMETRY,
-                             "geometry not supported by the tile kernels: upwind face of cell (%d,%d) "
-                             "direction %d axis %d differs from the reference addressing",
-                             cx, cy, i, axis);
+          differs = !same(a, b);
+        }
+        if (differs) {
+          s.phys_cells[(size_t)cy * s.nx + cx] = 1;
+          s.any_phys = true;
         }
       }
     }
   }
-  return LBM_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -400,6 +404,8 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.rdy = s.dt / s.p.dy;
   d.rdxs = s.spacing_dev;
   d.rdys = s.spacing_dev ? s.spacing_dev + s.nx : nullptr;
+  d.dxs = s.spacing_dev ? s.spacing_dev + s.nx + s.ny : nullptr;
+  d.dys = s.spacing_dev ? s.spacing_dev + 2 * s.nx + s.ny : nullptr;
   d.c = s.c;
   d.sRTc = std::sqrt(s.p.R * s.Tc);
   d.Tc = s.Tc;
@@ -457,7 +463,11 @@ static std::vector<TileRef> build_flux_tiles(const Solver& s, int shape) {
       // on the far side: be conservative on edge tiles
       const bool edge = x0 - H < 0 || y0 - H < 0 || x0 + TW + H > s.nx || y0 + TH + H > s.ny;
       if (edge && any_edge_solid) solid = true;
-      tiles.push_back(TileRef{0, tx, ty, solid ? kTileSolid : 0});
+      bool phys = false;
+      if (s.any_phys)
+        for (int y = y0; y < std::min(y0 + TH, s.ny) && !phys; y++)
+          for (int x = x0; x < std::min(x0 + TW, s.nx) && !phys; x++) phys = s.phys_cells[(size_t)y * s.nx + x];
+      tiles.push_back(TileRef{0, tx, ty, (solid ? kTileSolid : 0) | (phys ? kTilePhysical : 0)});
     }
   return tiles;
 }
@@ -549,8 +559,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
       s->ids_host[(size_t)y * nx + x] = id;
       s->ctype_host[(size_t)y * nx + x] = (uint8_t)s->props[id].type;
     }
-  int st = validate_geometry(*s);
-  if (st != LBM_OK) return st;
+  mark_physical_cells(*s);
   if (lbm_device_count() < 1) return fail_with(LBM_ERR_NOGPU, "no gfx950 device visible");
 
   const int align = 64;
@@ -605,6 +614,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   s->flux_shape = default_flux_shape(precision);
   const std::vector<TileRef>& tiles = solver_tiles(*s, s->flux_shape);
   s->ntiles = (int)tiles.size();
+  s->nphys = count_physical(tiles);
   s->nwalls = (int)walls.size();
   s->nptiles = (int)ptiles.size();
   HIP_TRY(hipMalloc((void**)&s->ptiles_dev, ptiles.size() * sizeof(TileRef)));
@@ -617,7 +627,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   }
   s->walls_host = std::move(walls);
   s->ptiles_host = std::move(ptiles);
-  st = upload_inst(*s);
+  const int st = upload_inst(*s);
   if (st != LBM_OK) return st;
   *out = reinterpret_cast<lbm_solver*>(s.release());
   return LBM_OK;
@@ -826,6 +836,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (nsteps == 0) return LBM_OK;
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
+  L.nphys = s.nphys;
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, s.graphs.get(), s.fails_dev, true);
   const int st2 = upload_inst(s);
@@ -892,6 +903,7 @@ extern "C" int lbm_step_noalloc(lbm_solver* h, int32_t nsteps, int32_t want_resi
   if (nsteps == 0) return LBM_OK;
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
+  L.nphys = s.nphys;
   prepare_launch(s.precision == LBM_PREC_FP64);
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, nullptr, s.fails_dev, false);
@@ -1101,9 +1113,9 @@ extern "C" int lbm_set_viscosity(lbm_solver* h, int32_t model, double mu_ref, do
 // 676-679): face fluxes weighted by their face length and the net divided by
 // the cell area, i.e. cFLUX with the cell's own dt/dx[x] and dt/dy[y] (the
 // two cells of a face share its length); dt from the smallest spacing
-// (PY/LBM.py:216), nsteps recomputed. Before the first step; refused for
-// decks with solid cells (the OpenCL wall closure is not the C++ one this
-// path follows).
+// (PY/LBM.py:216), nsteps recomputed. Walls keep the C++ closure
+// (cpuLBM.cpp:1502-1590) with the front end's face-length weights, which is
+// the C++ balance with the wall cell's own dx, dy. Before the first step.
 extern "C" int lbm_set_spacing(lbm_solver* h, const double* dx, const double* dy) {
   if (!h || !dx || !dy) return fail_with(LBM_ERR_ARG, "null argument");
   Solver& s = *reinterpret_cast<Solver*>(h);
@@ -1112,8 +1124,6 @@ extern "C" int lbm_set_spacing(lbm_solver* h, const double* dx, const double* dy
     if (!(dx[x] > 0) || !std::isfinite(dx[x])) return fail_with(LBM_ERR_ARG, "dx[%d] is not positive", x);
   for (int y = 0; y < s.ny; y++)
     if (!(dy[y] > 0) || !std::isfinite(dy[y])) return fail_with(LBM_ERR_ARG, "dy[%d] is not positive", y);
-  for (uint8_t t : s.ctype_host)
-    if (t == 2) return fail_with(LBM_ERR_ARG, "non-uniform spacing is not supported with solid cells");
   s.dxs.assign(dx, dx + s.nx);
   s.dys.assign(dy, dy + s.ny);
   const double mn = std::min(*std::min_element(s.dxs.begin(), s.dxs.end()),
@@ -1122,9 +1132,12 @@ extern "C" int lbm_set_spacing(lbm_solver* h, const double* dx, const double* dy
     s.dt = (s.p.cfl * mn) / (2 * s.c);
     steps_from_dt(s);
   }
-  std::vector<double> r(s.nx + s.ny);
+  // [dt/dx (nx) | dt/dy (ny) | dx (nx) | dy (ny)]
+  std::vector<double> r(2 * (s.nx + s.ny));
   for (int x = 0; x < s.nx; x++) r[x] = s.dt / s.dxs[x];
   for (int y = 0; y < s.ny; y++) r[s.nx + y] = s.dt / s.dys[y];
+  for (int x = 0; x < s.nx; x++) r[s.nx + s.ny + x] = s.dxs[x];
+  for (int y = 0; y < s.ny; y++) r[2 * s.nx + s.ny + y] = s.dys[y];
   if (!s.spacing_dev) HIP_TRY(hipMalloc((void**)&s.spacing_dev, r.size() * sizeof(double)));
   HIP_TRY(hipMemcpy(s.spacing_dev, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
   return upload_inst(s);
@@ -1185,6 +1198,7 @@ int uqlbm::batch_set_shape(Batch& b, int shape) {
   HIP_TRY(hipMalloc((void**)&b.tiles_dev, tiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   b.ntiles = (int)tiles.size();
+  b.nphys = count_physical(tiles);
   b.flux_shape = shape;
   return LBM_OK;
 }
@@ -1233,6 +1247,7 @@ uqlbm::Batch::~Batch() {
 int uqlbm::batch_step(Batch& b, int nsteps) {
   StepLaunch L{b.inst_dev,    b.tiles_dev, b.ntiles,    b.walls_dev, b.nwalls,
                b.ptiles_dev, b.nptiles,   b.rk_method, b.step_ctr,  b.flux_shape};
+  L.nphys = b.nphys;
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get(), b.fails_dev, true);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

@@ -49,6 +49,8 @@ struct Solver {
 
   std::vector<int> ids_host;        // [y][nx]
   std::vector<uint8_t> ctype_host;  // [y][nx]
+  std::vector<char> phys_cells;     // [y][nx]: net fluxes need the reference addressing
+  bool any_phys = false;
   std::vector<TileRef> tiles_shape[kFluxShapes];  // flux tiles per shape, built on first use
   std::vector<WallRef> walls_host;
   std::vector<TileRef> ptiles_host;
@@ -84,6 +86,7 @@ struct Solver {
   int* fails_dev = nullptr;         // failure step gathered for the host (one int)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
+  int nphys = 0;  // flux tiles flagged kTilePhysical
   size_t device_bytes = 0;
 
   int cur = 0;    // state buffer holding the current state
@@ -112,6 +115,7 @@ struct Batch {
   int flux_shape = 0;
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
+  int nphys = 0;  // flux tiles flagged kTilePhysical
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -127,9 +131,11 @@ struct Batch {
     fails_dev = o.fails_dev;
     flux_shape = o.flux_shape;
     graphs = std::move(o.graphs);
+    inst_host = o.inst_host;
     ntiles = o.ntiles;
     nwalls = o.nwalls;
     nptiles = o.nptiles;
+    nphys = o.nphys;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;

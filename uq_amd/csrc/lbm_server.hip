@@ -67,6 +67,7 @@ struct Server {
   TileRef* ptiles_dev = nullptr;
   size_t tiles_cap = 0, walls_cap = 0, ptiles_cap = 0;
   int ntiles = 0, nwalls = 0, nptiles = 0;
+  int nphys = 0;  // flux tiles flagged kTilePhysical
   bool lists_dirty = true;
   int* step_ctr = nullptr;
   int* fails_dev = nullptr;  // failure steps of the active requests, gathered for one host copy
@@ -146,6 +147,7 @@ static int rebuild_lists(Server& sv) {
                           sv.stream));
   SV_TRY(hipStreamSynchronize(sv.stream));
   sv.ntiles = (int)tiles.size();
+  sv.nphys = count_physical(tiles);
   sv.nwalls = (int)walls.size();
   sv.nptiles = (int)ptiles.size();
   sv.lists_dirty = false;
@@ -342,6 +344,7 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     for (auto& a : sv.active) members.push_back(a.solver);
     StepLaunch L{sv.table_dev,  sv.tiles_dev, sv.ntiles,     sv.walls_dev, sv.nwalls,
                  sv.ptiles_dev, sv.nptiles,   sv.rk_method, sv.step_ctr,  sv.flux_shape};
+    L.nphys = sv.nphys;
     if (track) {
       L.res_partial = sv.res_partial;
       L.ninst = (int)members.size();
