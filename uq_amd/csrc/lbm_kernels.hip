@@ -109,11 +109,13 @@ __device__ __forceinline__ void flag_failure(const InstDev& in, int step, double
 }
 
 // Tolerance of the equilibrium mass check: the reference's 1e-6 for FP64
-// arithmetic; for FP32 equilibria the rounding bound of the FP32 sum is added
-// so that rounding alone never trips it.
-template <typename TC>
+// equilibria; for the FP32 equilibria of FP32 storage (TE = float: the
+// deviations from the reference equilibrium, whose absolute values sum to
+// sum_abs) the rounding bound of the FP32 evaluation and sum is added, so
+// that rounding alone never trips it.
+template <typename TE>
 __device__ __forceinline__ double eq_check_tol(double sum_abs) {
-  return std::is_same<TC, double>::value ? 1e-6 : 1e-6 + 16.0 * 5.96e-8 * sum_abs;
+  return std::is_same<TE, double>::value ? 1e-6 : 1e-6 + 16.0 * 5.96e-8 * sum_abs;
 }
 
 // ---------------------------------------------------------------------------
@@ -223,15 +225,18 @@ __device__ __forceinline__ double hval(const CellState<TS>& c) {
   }
 }
 
-// Equilibrium of direction I in the stored form: FP64 storage the closed
-// form in FP64; FP32 storage its deviation from the reference equilibrium,
-// evaluated in FP32 in the deviation form (eq_dev). With SUM, the value
+// Equilibrium of direction I in the stored form. TE is the type the
+// equilibria are evaluated in: FP64 (the closed form; FP32 storage then
+// takes the reference equilibrium off) or, for FP32 storage of decks near
+// their reference equilibrium, FP32 in the deviation form (eq_dev), whose
+// every term is of the deviation's size. With SUM, the stored-form value
 // joins the failure-check sums (FP32 storage: sums of deviations, to which
 // failure_mass adds the reference's mass).
-template <int I, typename TS, bool SUM, typename TC>
+template <int I, typename TS, bool SUM, typename TE, typename TC>
 __device__ __forceinline__ void eq_stored(const InstDev& in, const EqIn<TC>& q, const EqDev& qd, TC& fe, TC& he,
                                           double& sumfe, double& sumabs) {
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (sizeof(TE) == 4) {
+    static_assert(sizeof(TS) == 4, "FP32 equilibria only for FP32 storage");
     float a, b;
     eq_dev<I>(qd, in.eqref, a, b);
     fe = TC(a);
@@ -239,6 +244,10 @@ __device__ __forceinline__ void eq_stored(const InstDev& in, const EqIn<TC>& q, 
   } else {
     (void)qd;
     eq_dir<I>(q, fe, he);
+    if constexpr (sizeof(TS) == 4) {
+      fe -= TC(in.ref[I]);
+      he -= TC(in.ref[kQ + I]);
+    }
   }
   if constexpr (SUM) {
     sumfe += double(fe);
@@ -323,7 +332,7 @@ __device__ __forceinline__ PointCell point_cell(const InstDev& in, const TileRef
 // ---------------------------------------------------------------------------
 // k_s12: RK3 stages 1 and 2, pointwise -> f2.
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int... Is>
+template <typename TS, typename TC, typename TE, int... Is>
 __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const double* mom, TS* dst, long long o,
                                          int step, Seq<Is...>) {
   const long long P = in.plane;
@@ -338,7 +347,7 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC fe, he, f2, h2;
-    eq_stored<I, TS, true>(in, r.q, qd, fe, he, sumfe, sumabs);
+    eq_stored<I, TS, true, TE>(in, r.q, qd, fe, he, sumfe, sumabs);
     s12<I>(r, hdt, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f2, h2);
     // streaming stores: f2 and, below, f3 and G are written once per step
     // and read back only by the next kernel, long after L2 has turned over
@@ -346,10 +355,10 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const
     __builtin_nontemporal_store(TS(h2), &dst[P * (kQ + I) + o]);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
+  flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TE>(sumabs));
 }
 
-template <typename TS, typename TC>
+template <typename TS, typename TC, typename TE>
 __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
                                              const int* stepp) {
   const int step = *stepp;
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
       for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
       continue;
     }
-    s12_cell<TS, TC>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
+    s12_cell<TS, TC, TE>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
@@ -382,7 +391,7 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
 // and the new state is G - dF(f3)/2; A = (e.u/thf)(feq - f). (P + dF2/2 is
 // f - dF2/2 + dt/2 (feq2 - f2)/tf2 since af2 = dt/(2 tf2).)
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int... Is>
+template <typename TS, typename TC, typename TE, int... Is>
 __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb,
                                         TS* f3b, TS* out, double* mom_out, long long o, int step, Seq<Is...>) {
   const long long P = in.plane;
@@ -427,7 +436,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     }
     const TC fv = TC(fval<I>(cs)), hv = TC(hval<I>(cs));
     TC fe, he, f2, h2;
-    eq_stored<I, TS, false>(in, r.q, qd1, fe, he, sumfe, sumabs);
+    eq_stored<I, TS, false, TE>(in, r.q, qd1, fe, he, sumfe, sumabs);
     s12<I>(r, hdt, fv, hv, fe, he, f2, h2);
     const TC D = fe - f2;
     const TC eu2 = TC(lat_x(I)) * r.cU + TC(lat_y(I)) * r.cV;
@@ -435,7 +444,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     const TC Pf = (fv - dF) + r.af * D;
     const TC Qh = ((hv - dH) - hdt * A2) + r.ah * (he - h2);
     TC fe3, he3;
-    eq_stored<I, TS, true>(in, q3, qd3, fe3, he3, sumfe, sumabs);
+    eq_stored<I, TS, true, TE>(in, q3, qd3, fe3, he3, sumfe, sumabs);
     const TC f3 = (Pf + af3 * fe3) * iaf3;
     const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
     const TC A3 = (eu3 * ithf3) * (fe3 - f3);
@@ -451,7 +460,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     if constexpr (sizeof(TS) == 4) __builtin_amdgcn_sched_barrier(0);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m3.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
+  flag_failure(in, step, m3.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TE>(sumabs));
 }
 
 // Frozen cell (permanent or solid) of a pointwise stage: the stored values
@@ -474,7 +483,7 @@ __device__ __forceinline__ void copy_frozen(const InstDev& in, const TS* src, TS
 // instantiation otherwise takes 262 registers (one wave per SIMD); capping
 // it at 256 costs a few spills and hides twice the memory latency (4096^2
 // FP64: 5.9 -> 4.2 ms per launch).
-template <typename TS, typename TC>
+template <typename TS, typename TC, typename TE>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k_s3(
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   const int step = *stepp;
@@ -495,7 +504,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k
       copy_frozen(in, src, out, cur, nxt, c.o);
       continue;
     }
-    s3_cell<TS, TC>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), f3b, out, in.mom[nxt], c.o, step,
+    s3_cell<TS, TC, TE>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), f3b, out, in.mom[nxt], c.o, step,
                     MakeSeq<kQ>::type{});
   }
 }
@@ -538,7 +547,7 @@ __device__ __forceinline__ void s1(const Rec1<TC>& r, TC fv, TC hv, TC fe, TC he
   h1 = (hv - r.dt * eu * ((fe - f1) * r.ithf) + r.bh * he) * r.ibh;
 }
 
-template <typename TS, typename TC, int... Is>
+template <typename TS, typename TC, typename TE, int... Is>
 __device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, const double* mom, TS* dst, long long o,
                                         int step, Seq<Is...>) {
   const long long P = in.plane;
@@ -552,16 +561,16 @@ __device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, const 
   auto one = [&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     TC fe, he, f1, h1;
-    eq_stored<I, TS, true>(in, r.q, qd, fe, he, sumfe, sumabs);
+    eq_stored<I, TS, true, TE>(in, r.q, qd, fe, he, sumfe, sumabs);
     s1<I>(r, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f1, h1);
     dst[P * I + o] = TS(f1);
     dst[P * (kQ + I) + o] = TS(h1);
   };
   (one(std::integral_constant<int, Is>{}), ...);
-  flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
+  flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TE>(sumabs));
 }
 
-template <typename TS, typename TC>
+template <typename TS, typename TC, typename TE>
 __global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
                                             const int* stepp) {
   const int step = *stepp;
@@ -579,13 +588,13 @@ __global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef*
       for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
       continue;
     }
-    s1_cell<TS, TC>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
+    s1_cell<TS, TC, TE>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
 // Final update of the one-flux-stage steppers, METHOD 0 (RK1, cpuLBM.cpp:
 // 1696, 1707), 2 (EULER, :2155, :2165), 3 (EULER_EQ, :2245, :2255).
-template <int METHOD, typename TS, typename TC, int... Is>
+template <int METHOD, typename TS, typename TC, typename TE, int... Is>
 __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb, TS* out,
                                          double* mom_out, long long o, int step, Seq<Is...>) {
   const long long P = in.plane;
@@ -609,13 +618,13 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
     TC nf, nh;
     if constexpr (METHOD == 0) {
       TC fe, he, f1, h1;
-      eq_stored<I, TS, false>(in, r.q, qd, fe, he, sumfe, sumabs);
+      eq_stored<I, TS, false, TE>(in, r.q, qd, fe, he, sumfe, sumabs);
       s1<I>(r, fv, hv, fe, he, f1, h1);
       nf = fv - dF + r.bf * (fe - f1);
       nh = hv - dH + r.bh * (he - h1) - ((r.dt * eu) * r.ithf) * (fe - f1);
     } else {
       TC fe, he;
-      eq_stored<I, TS, true>(in, r.q, qd, fe, he, sumfe, sumabs);
+      eq_stored<I, TS, true, TE>(in, r.q, qd, fe, he, sumfe, sumabs);
       if constexpr (METHOD == 2) {
         nf = fv - dF + r.bf * (fe - fv);
         nh = hv - dH + r.bh * (he - hv) - ((r.dt * eu) * r.ithf) * (fe - fv);
@@ -640,10 +649,10 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
   } else {
     (void)mom_out;
   }
-  if (METHOD != 0) flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TC>(sumabs));
+  if (METHOD != 0) flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TE>(sumabs));
 }
 
-template <typename TS, typename TC, int METHOD>
+template <typename TS, typename TC, typename TE, int METHOD>
 __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
                                              const int* stepp) {
   const int step = *stepp;
@@ -660,7 +669,7 @@ __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef
       copy_frozen(in, src, out, cur, nxt, c.o);
       continue;
     }
-    fin_cell<METHOD, TS, TC>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), out, in.mom[nxt], c.o,
+    fin_cell<METHOD, TS, TC, TE>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), out, in.mom[nxt], c.o,
                              step, MakeSeq<kQ>::type{});
   }
 }
@@ -1847,12 +1856,15 @@ static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 // Persistent grid sizes of every step kernel of one storage/compute type and
 // flux tile shape, computed once (before any stream capture).
 // Compute type of the pointwise stage kernels and the wall closure: FP64
-// for both storage types. FP32 storage evaluates the equilibria in FP32 as
-// deviations from the reference equilibrium (eq_dev), whose every term is
-// of the deviation's size, and forms the stage updates in FP64 and rounds
-// them once, on the store: stage updates in FP32 (five or more roundings of
-// the deviation per stored value) triple the error over 5000 steps.
-template <typename TS>
+// for both storage types; the stage updates of FP32 storage are formed in
+// FP64 and rounded once, on the store (stage updates in FP32, five or more
+// roundings of the deviation per stored value, triple the error over 5000
+// steps of the UQ decks). The equilibria of FP32 storage are evaluated in
+// FP32 in the deviation form when every instance of the launch group starts
+// near its reference equilibrium (StepLaunch::eq32), else in FP64: on a
+// shock deck, whose deviations are of the order of the state itself, FP32
+// equilibria seed the scheme's unstable mode about as strongly as FP32
+// states do and the run ends earlier.
 using TP = double;
 using TW = double;
 
@@ -1861,36 +1873,40 @@ using TW = double;
 template <typename TS>
 constexpr bool kMom3 = sizeof(TS) == 4;
 
-template <typename TS, typename TC, typename Sh>
+template <typename TS, typename TC, typename TE, typename Sh>
 struct StepGrids {
   int s12, flux2, s3, flux3, s1, flux0, fluxe, fin0, fin2, fin3;
   static const StepGrids& get() {
-    static const StepGrids g = {grid_for((const void*)k_s12<TS, TP<TS>>, kNT),
+    static const StepGrids g = {grid_for((const void*)k_s12<TS, TP, TE>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s3<TS, TP<TS>>, kNT),
+                                grid_for((const void*)k_s3<TS, TP, TE>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s1<TS, TP<TS>>, kNT),
+                                grid_for((const void*)k_s1<TS, TP, TE>, kNT),
                                 grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT),
                                 grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_fin<TS, TP<TS>, 0>, kNT),
-                                grid_for((const void*)k_fin<TS, TP<TS>, 2>, kNT),
-                                grid_for((const void*)k_fin<TS, TP<TS>, 3>, kNT)};
+                                grid_for((const void*)k_fin<TS, TP, TE, 0>, kNT),
+                                grid_for((const void*)k_fin<TS, TP, TE, 2>, kNT),
+                                grid_for((const void*)k_fin<TS, TP, TE, 3>, kNT)};
     return g;
   }
 };
 
-template <typename TS, typename TC>
+template <typename TS, typename TC, typename TE>
 static void prepare_shapes() {
-  (void)StepGrids<TS, TC, Shape0>::get();
-  (void)StepGrids<TS, TC, Shape1>::get();
-  (void)StepGrids<TS, TC, Shape2>::get();
-  (void)StepGrids<TS, TC, Shape3>::get();
-  (void)StepGrids<TS, TC, Shape4>::get();
+  (void)StepGrids<TS, TC, TE, Shape0>::get();
+  (void)StepGrids<TS, TC, TE, Shape1>::get();
+  (void)StepGrids<TS, TC, TE, Shape2>::get();
+  (void)StepGrids<TS, TC, TE, Shape3>::get();
+  (void)StepGrids<TS, TC, TE, Shape4>::get();
 }
 
 void prepare_launch(int fp64) {
-  if (fp64) prepare_shapes<double, double>();
-  else prepare_shapes<float, float>();
+  if (fp64) {
+    prepare_shapes<double, double, double>();
+  } else {
+    prepare_shapes<float, float, float>();
+    prepare_shapes<float, float, double>();
+  }
 }
 
 // The physical-addressing net fluxes of the flagged tiles, after the flux
@@ -1902,15 +1918,15 @@ static void launch_phys(const StepLaunch& L, hipStream_t s, int par) {
                                                                                   L.step_ctr);
 }
 
-template <typename TS, typename TC, typename Sh>
+template <typename TS, typename TC, typename TE, typename Sh>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
-  const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
+  const StepGrids<TS, TC, TE, Sh>& G = StepGrids<TS, TC, TE, Sh>::get();
   const int g0 = G.s12, g1 = G.flux2, g2 = G.s3, g3 = G.flux3;
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
-  k_s12<TS, TP<TS>><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_s12<TS, TP, TE><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(0, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
@@ -1922,7 +1938,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
   launch_phys<TS, TC, 0, true, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_s3<TS, TP<TS>><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_s3<TS, TP, TE><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
@@ -1936,18 +1952,18 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
 }
 
 // RKMETHOD 0, 2, 3: one flux stage.
-template <typename TS, typename TC, int METHOD, typename Sh>
+template <typename TS, typename TC, typename TE, int METHOD, typename Sh>
 static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
   constexpr int FM = METHOD == 0 ? 0 : 2;
-  const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
+  const StepGrids<TS, TC, TE, Sh>& G = StepGrids<TS, TC, TE, Sh>::get();
   const int g0 = G.s1, g1 = FM == 0 ? G.flux0 : G.fluxe;
   const int g2 = METHOD == 0 ? G.fin0 : (METHOD == 2 ? G.fin2 : G.fin3);
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if constexpr (METHOD == 0) {
     if (tm) tm->begin(0, s);
-    k_s1<TS, TP<TS>><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+    k_s1<TS, TP, TE><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
     if (tm) tm->end(0, s);
   }
   if (L.nwalls) {
@@ -1960,18 +1976,29 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
   launch_phys<TS, TC, FM, false, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_fin<TS, TP<TS>, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  k_fin<TS, TP, TE, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
   if (tm) tm->end(3, s);
+}
+
+template <typename TS, typename TC, typename TE, typename Sh>
+static void launch_stepper(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
+  switch (L.rk_method) {
+    case 0: launch_one_stage<TS, TC, TE, 0, Sh>(L, s, par, tm); break;
+    case 2: launch_one_stage<TS, TC, TE, 2, Sh>(L, s, par, tm); break;
+    case 3: launch_one_stage<TS, TC, TE, 3, Sh>(L, s, par, tm); break;
+    default: launch_rk3<TS, TC, TE, Sh>(L, s, par, tm); break;
+  }
 }
 
 template <typename TS, typename TC, typename Sh>
 static void launch_step_shape(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
-  switch (L.rk_method) {
-    case 0: launch_one_stage<TS, TC, 0, Sh>(L, s, par, tm); break;
-    case 2: launch_one_stage<TS, TC, 2, Sh>(L, s, par, tm); break;
-    case 3: launch_one_stage<TS, TC, 3, Sh>(L, s, par, tm); break;
-    default: launch_rk3<TS, TC, Sh>(L, s, par, tm); break;
+  if constexpr (sizeof(TS) == 4) {
+    if (L.eq32) {
+      launch_stepper<TS, TC, float, Sh>(L, s, par, tm);
+      return;
+    }
   }
+  launch_stepper<TS, TC, double, Sh>(L, s, par, tm);
 }
 
 // Advances the step counter of a launch sequence: the kernels of step k read

@@ -28,6 +28,8 @@ struct StepLaunch {
                                     // some instance tracks its residual; null otherwise
   int ninst = 0;                    // instances in the table (for the residual tracking)
   int nphys = 0;                    // tiles flagged kTilePhysical in `tiles` (k_phys runs when > 0)
+  int eq32 = 0;                     // FP32 storage: every instance starts near its reference
+                                    // equilibrium, equilibria in FP32 (deviation form)
 };
 
 // Flux tiles of a list flagged kTilePhysical.
@@ -41,7 +43,7 @@ inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
   return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
          a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
          a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape && a.res_partial == b.res_partial && a.ninst == b.ninst &&
-         a.nphys == b.nphys;
+         a.nphys == b.nphys && a.eq32 == b.eq32;
 }
 
 // Optional per-kernel timing with HIP events recorded on the launch stream

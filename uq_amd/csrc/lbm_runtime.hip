@@ -314,6 +314,25 @@ static void eq_all(const EqIn<double>& q, double* f, double* h, std::integer_seq
   ((eq_dir<Is>(q, f[Is], h[Is])), ...);
 }
 
+// FP32 storage: whether every fluid or permanent cell starts near the
+// reference equilibrium (density and temperature within 25 % of the
+// reference's, speed within half of sqrt(R Tc)). Launch groups made of such
+// instances evaluate their equilibria in FP32 in the deviation form, whose
+// terms are then small; others in FP64.
+static bool near_reference(const Solver& s) {
+  std::vector<char> used(s.props.size(), 0);
+  for (int id : s.ids_host) used[id] = 1;
+  const double sc = std::sqrt(s.p.R * s.Tc);
+  for (size_t r = 0; r < s.props.size(); r++) {
+    const PropRow& p = s.props[r];
+    if (!used[r] || p.type == 2) continue;
+    if (std::fabs(p.rho - s.ref_rho) > 0.25 * s.ref_rho) return false;
+    if (std::fabs(p.T / s.Tc - s.ref_t) > 0.25 * s.ref_t) return false;
+    if (std::hypot(p.ux, p.uy) > 0.5 * sc) return false;
+  }
+  return true;
+}
+
 static void set_reference(Solver& s) {
   double rho = 0, T = 0;
   long long n = 0;
@@ -575,6 +594,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
       HIP_TRY(hipMemset(*m, 0, plane * 4 * sizeof(double)));
     }
     set_reference(*s);
+    s->eq32 = near_reference(*s);
   }
   HIP_TRY(hipMalloc((void**)&s->ctype, plane));
   HIP_TRY(hipMalloc((void**)&s->alpha, plane * sizeof(double)));
@@ -837,6 +857,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   L.nphys = s.nphys;
+  L.eq32 = s.eq32 ? 1 : 0;
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, s.graphs.get(), s.fails_dev, true);
   const int st2 = upload_inst(s);
@@ -904,6 +925,7 @@ extern "C" int lbm_step_noalloc(lbm_solver* h, int32_t nsteps, int32_t want_resi
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   L.nphys = s.nphys;
+  L.eq32 = s.eq32 ? 1 : 0;
   prepare_launch(s.precision == LBM_PREC_FP64);
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, nullptr, s.fails_dev, false);
@@ -1199,6 +1221,8 @@ int uqlbm::batch_set_shape(Batch& b, int shape) {
   HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   b.ntiles = (int)tiles.size();
   b.nphys = count_physical(tiles);
+  b.eq32 = 1;
+  for (Solver* m : b.members) b.eq32 &= m->eq32 ? 1 : 0;
   b.flux_shape = shape;
   return LBM_OK;
 }
@@ -1248,6 +1272,7 @@ int uqlbm::batch_step(Batch& b, int nsteps) {
   StepLaunch L{b.inst_dev,    b.tiles_dev, b.ntiles,    b.walls_dev, b.nwalls,
                b.ptiles_dev, b.nptiles,   b.rk_method, b.step_ctr,  b.flux_shape};
   L.nphys = b.nphys;
+  L.eq32 = b.eq32;
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get(), b.fails_dev, true);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

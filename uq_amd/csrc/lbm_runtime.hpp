@@ -63,6 +63,7 @@ struct Solver {
   double* mom[2] = {nullptr, nullptr};  // FP32 storage: conserved moments of state[k] (4 planes)
   double ref[kPlanes] = {};             // FP32 storage: reference equilibrium (deviation origin)
   double ref_rho = 0, ref_t = 0;        // its density and T/Tc
+  bool eq32 = false;                    // FP32 storage near the reference: FP32 equilibria
   uint8_t* ctype = nullptr;
   double* alpha = nullptr;
   int* fail = nullptr;
@@ -79,7 +80,7 @@ struct Solver {
   int visc_model = 0;               // viscosity law (lbm_set_viscosity)
   double mu_ref = 0, T_ref_v = 0, visc_param = 0;
   std::vector<double> dxs, dys;     // non-uniform spacing (lbm_set_spacing), empty when uniform
-  double* spacing_dev = nullptr;    // dt/dx[nx] then dt/dy[ny]
+  double* spacing_dev = nullptr;    // dt/dx[nx], dt/dy[ny], dx[nx], dy[ny]
   double* totals_dev = nullptr;     // conserved-totals reduction scratch (allocated on first use)
   double* resid_dev = nullptr;      // residual row sums, ny + 1 doubles (allocated on first use)
   double* res_hist = nullptr;       // served with a tolerance: per-step residual history (kResHist)
@@ -116,6 +117,7 @@ struct Batch {
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   int nphys = 0;  // flux tiles flagged kTilePhysical
+  int eq32 = 0;   // every member near its reference equilibrium (FP32 storage)
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -136,6 +138,7 @@ struct Batch {
     nwalls = o.nwalls;
     nptiles = o.nptiles;
     nphys = o.nphys;
+    eq32 = o.eq32;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;
