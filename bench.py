@@ -250,14 +250,16 @@ def plate_leg(steps: int) -> dict:
 def riemann_leg(n: int, steps: int, time_steps: int, shape, cfl: float) -> dict:
     """Config 5: the four-quadrant Riemann problem at n^2 (the reference's
     Lax-Liu set-up, physical air viscosity) in FP64 and mixed precision:
-    `time_steps` steps timed, then continued to `steps` and the relative L2
-    of density and temperature between the two precisions taken there. CFL
-    0.1 by default: 2000 steps then reach the end time of the reference
-    generator's own 1000 steps at CFL 0.2 (makeLaxLiu3.py:22, 26); at CFL
-    0.2 the scheme itself trips its equilibrium check near step 1050 (the
-    oracle too). The step where each run stops is reported."""
+    `time_steps` steps of each timed, then both continued side by side to
+    `steps`, with the relative L2 of density and temperature between the two
+    precisions taken at every 500 steps. CFL 0.1 by default: 2000 steps then
+    reach the end time of the reference generator's own 1000 steps at CFL
+    0.2 (makeLaxLiu3.py:22, 26); at CFL 0.2 the scheme itself trips its
+    equilibrium check near step 1050 (the oracle too), and the difference
+    between the precisions grows with the deck's unstable mode. The step
+    where each run stops is reported."""
     out = {"n": n, "cfl": cfl, "steps_timed": time_steps, "steps_requested": steps}
-    fields = {}
+    runs = {}
     for prec in ("fp64", "mixed"):
         if shape is not None:
             lbm.set_flux_shape(prec, shape)
@@ -273,19 +275,32 @@ def riemann_leg(n: int, steps: int, time_steps: int, shape, cfl: float) -> dict:
         rs.step(3)
         rk = lbm.last_timing()["kernels"]
         lbm.set_kernel_timing(False)
-        st_full = rs.step(max(0, steps - time_steps - 3))
-        stopped = rs.info()["step"]
-        fields[prec] = rs.get_macro()
-        rs.close()
+        runs[prec] = rs
         mlups = rc.cells * time_steps / sec / 1e6
         es = 8 if prec == "fp64" else 4
         out[prec] = {"mlups": round(mlups, 2), "status_timed": st,
                      "step_roofline_frac_1040B": round(mlups * MODEL_BYTES_FP64 / 1e3 / HBM_PEAK_GBS, 4),
                      "roofline": roofline(rk, rc.cells, es, f"riemann{n}_{prec}", shape),
-                     "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in rk.items()},
-                     "full_run": {"status": st_full, "stopped_at_step": stopped}}
-    out["mixed_vs_fp64_rel_l2"] = {"steps": steps, **{
-        k: float(f"{lbm.rel_l2(fields['mixed'][k], fields['fp64'][k]):.3e}") for k in ("rho", "T")}}
+                     "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in rk.items()}}
+    cmp = []
+    for cp in [k for k in range(500, steps + 1, 500)] + ([steps] if steps % 500 else []):
+        fields, status = {}, {}
+        for prec, rs in runs.items():
+            todo = cp - rs.info()["step"]
+            status[prec] = rs.step(todo) if todo > 0 else 0
+            fields[prec] = rs.get_macro()
+        row = {"step": cp}
+        if all(v == 0 for v in status.values()):
+            row.update({k: float(f"{lbm.rel_l2(fields['mixed'][k], fields['fp64'][k]):.3e}") for k in ("rho", "T")})
+        else:
+            row["stopped"] = {p: rs.info()["step"] for p, rs in runs.items()}
+        cmp.append(row)
+        if "stopped" in row:
+            break
+    out["mixed_vs_fp64_rel_l2"] = cmp
+    for prec, rs in runs.items():
+        out[prec]["full_run"] = {"stopped_at_step": rs.info()["step"], "failed": bool(rs.info()["step"] < steps)}
+        rs.close()
     return out
 
 

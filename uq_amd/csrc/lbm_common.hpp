@@ -58,6 +58,7 @@ struct InstDev {
                        // momentum, energy)
   double rho0, t0;     // FP32 storage: density and T/Tc of the reference equilibrium
   float eqref[8];      // FP32 storage: the base polynomials at rest at t0
+  int eq32;            // FP32 storage near the reference: equilibria in FP32 (eq_dev), else FP64
                        // (f0, f1, f5, f9 per unit density; h0, h1, h5, h9)
   double i2c;          // 1/(2c): momentum correction per axis population pair
   // physics (reference setupVars, cpuLBM.cpp:360-450)

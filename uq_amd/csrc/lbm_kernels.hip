@@ -329,6 +329,16 @@ __device__ __forceinline__ PointCell point_cell(const InstDev& in, const TileRef
   return c;
 }
 
+// Whether the pointwise kernel instantiated for equilibrium type TE takes
+// instance `in`: FP32 storage picks the equilibrium precision per instance
+// (InstDev::eq32), and a launch group with both kinds launches each
+// pointwise kernel once per kind.
+template <typename TS, typename TE>
+__device__ __forceinline__ bool eq_kernel_for(const InstDev& in) {
+  if constexpr (sizeof(TS) == 4) return (in.eq32 != 0) == (sizeof(TE) == 4);
+  else return true;
+}
+
 // ---------------------------------------------------------------------------
 // k_s12: RK3 stages 1 and 2, pointwise -> f2.
 // ---------------------------------------------------------------------------
@@ -366,6 +376,7 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
+    if (!eq_kernel_for<TS, TE>(in)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par;
@@ -491,6 +502,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
+    if (!eq_kernel_for<TS, TE>(in)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
@@ -578,6 +590,7 @@ __global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef*
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
+    if (!eq_kernel_for<TS, TE>(in)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par;
@@ -660,6 +673,7 @@ __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef
     const TileRef tr = tiles[t];
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
+    if (!eq_kernel_for<TS, TE>(in)) continue;
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
@@ -1860,11 +1874,11 @@ static int blocks_for(int ntiles, int g) { return ntiles <= g ? ntiles : g; }
 // FP64 and rounded once, on the store (stage updates in FP32, five or more
 // roundings of the deviation per stored value, triple the error over 5000
 // steps of the UQ decks). The equilibria of FP32 storage are evaluated in
-// FP32 in the deviation form when every instance of the launch group starts
-// near its reference equilibrium (StepLaunch::eq32), else in FP64: on a
-// shock deck, whose deviations are of the order of the state itself, FP32
-// equilibria seed the scheme's unstable mode about as strongly as FP32
-// states do and the run ends earlier.
+// FP32 in the deviation form for instances that start near their reference
+// equilibrium (InstDev::eq32), else in FP64: on a shock deck, whose
+// deviations are of the order of the state itself, FP32 equilibria seed the
+// scheme's unstable mode about as strongly as FP32 states do and the run
+// ends earlier.
 using TP = double;
 using TW = double;
 
@@ -1873,39 +1887,60 @@ using TW = double;
 template <typename TS>
 constexpr bool kMom3 = sizeof(TS) == 4;
 
-template <typename TS, typename TC, typename TE, typename Sh>
+// Pointwise kernel grids are per equilibrium precision: [0] FP64, [1] FP32
+// (FP32 storage only).
+template <typename TS, typename TC, typename Sh>
 struct StepGrids {
-  int s12, flux2, s3, flux3, s1, flux0, fluxe, fin0, fin2, fin3;
+  int s12[2] = {0, 0}, s3[2] = {0, 0}, s1[2] = {0, 0}, fin0[2] = {0, 0}, fin2[2] = {0, 0}, fin3[2] = {0, 0};
+  int flux2, flux3, flux0, fluxe;
+  template <typename TE>
+  void pointwise(int k) {
+    s12[k] = grid_for((const void*)k_s12<TS, TP, TE>, kNT);
+    s3[k] = grid_for((const void*)k_s3<TS, TP, TE>, kNT);
+    s1[k] = grid_for((const void*)k_s1<TS, TP, TE>, kNT);
+    fin0[k] = grid_for((const void*)k_fin<TS, TP, TE, 0>, kNT);
+    fin2[k] = grid_for((const void*)k_fin<TS, TP, TE, 2>, kNT);
+    fin3[k] = grid_for((const void*)k_fin<TS, TP, TE, 3>, kNT);
+  }
+  static StepGrids make() {
+    StepGrids g;
+    g.template pointwise<double>(0);
+    if constexpr (sizeof(TS) == 4) g.template pointwise<float>(1);
+    g.flux2 = grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT);
+    g.flux3 = grid_for((const void*)FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn, Sh::NT);
+    g.flux0 = grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT);
+    g.fluxe = grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT);
+    return g;
+  }
   static const StepGrids& get() {
-    static const StepGrids g = {grid_for((const void*)k_s12<TS, TP, TE>, kNT),
-                                grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s3<TS, TP, TE>, kNT),
-                                grid_for((const void*)FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_s1<TS, TP, TE>, kNT),
-                                grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT),
-                                grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT),
-                                grid_for((const void*)k_fin<TS, TP, TE, 0>, kNT),
-                                grid_for((const void*)k_fin<TS, TP, TE, 2>, kNT),
-                                grid_for((const void*)k_fin<TS, TP, TE, 3>, kNT)};
+    static const StepGrids g = make();
     return g;
   }
 };
 
-template <typename TS, typename TC, typename TE>
+template <typename TS, typename TC>
 static void prepare_shapes() {
-  (void)StepGrids<TS, TC, TE, Shape0>::get();
-  (void)StepGrids<TS, TC, TE, Shape1>::get();
-  (void)StepGrids<TS, TC, TE, Shape2>::get();
-  (void)StepGrids<TS, TC, TE, Shape3>::get();
-  (void)StepGrids<TS, TC, TE, Shape4>::get();
+  (void)StepGrids<TS, TC, Shape0>::get();
+  (void)StepGrids<TS, TC, Shape1>::get();
+  (void)StepGrids<TS, TC, Shape2>::get();
+  (void)StepGrids<TS, TC, Shape3>::get();
+  (void)StepGrids<TS, TC, Shape4>::get();
 }
 
 void prepare_launch(int fp64) {
-  if (fp64) {
-    prepare_shapes<double, double, double>();
+  if (fp64) prepare_shapes<double, double>();
+  else prepare_shapes<float, float>();
+}
+
+// Calls launch(TE{}, k) once per equilibrium precision present in the group
+// (StepLaunch::eq_kinds; k indexes the StepGrids arrays).
+template <typename TS, typename F>
+static void per_eq_kind(const StepLaunch& L, F&& launch) {
+  if constexpr (sizeof(TS) == 4) {
+    if (L.eq_kinds & 2) launch(float{}, 1);
+    if (L.eq_kinds & 1) launch(double{}, 0);
   } else {
-    prepare_shapes<float, float, float>();
-    prepare_shapes<float, float, double>();
+    launch(double{}, 0);
   }
 }
 
@@ -1918,15 +1953,18 @@ static void launch_phys(const StepLaunch& L, hipStream_t s, int par) {
                                                                                   L.step_ctr);
 }
 
-template <typename TS, typename TC, typename TE, typename Sh>
+template <typename TS, typename TC, typename Sh>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
-  const StepGrids<TS, TC, TE, Sh>& G = StepGrids<TS, TC, TE, Sh>::get();
-  const int g0 = G.s12, g1 = G.flux2, g2 = G.s3, g3 = G.flux3;
+  const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
+  const int g1 = G.flux2, g3 = G.flux3;
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
-  k_s12<TS, TP, TE><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  per_eq_kind<TS>(L, [&](auto te, int k) {
+    k_s12<TS, TP, decltype(te)><<<blocks_for(L.nptiles, G.s12[k]), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par,
+                                                                                step);
+  });
   if (tm) tm->end(0, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
@@ -1938,7 +1976,10 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
   launch_phys<TS, TC, 0, true, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_s3<TS, TP, TE><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  per_eq_kind<TS>(L, [&](auto te, int k) {
+    k_s3<TS, TP, decltype(te)><<<blocks_for(L.nptiles, G.s3[k]), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par,
+                                                                              step);
+  });
   if (tm) tm->end(3, s);
   if (L.nwalls) {
     if (tm) tm->begin(1, s);
@@ -1952,18 +1993,21 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
 }
 
 // RKMETHOD 0, 2, 3: one flux stage.
-template <typename TS, typename TC, typename TE, int METHOD, typename Sh>
+template <typename TS, typename TC, int METHOD, typename Sh>
 static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
   constexpr int FM = METHOD == 0 ? 0 : 2;
-  const StepGrids<TS, TC, TE, Sh>& G = StepGrids<TS, TC, TE, Sh>::get();
-  const int g0 = G.s1, g1 = FM == 0 ? G.flux0 : G.fluxe;
-  const int g2 = METHOD == 0 ? G.fin0 : (METHOD == 2 ? G.fin2 : G.fin3);
+  const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
+  const int g1 = FM == 0 ? G.flux0 : G.fluxe;
+  const int* g2 = METHOD == 0 ? G.fin0 : (METHOD == 2 ? G.fin2 : G.fin3);
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if constexpr (METHOD == 0) {
     if (tm) tm->begin(0, s);
-    k_s1<TS, TP, TE><<<blocks_for(L.nptiles, g0), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+    per_eq_kind<TS>(L, [&](auto te, int k) {
+      k_s1<TS, TP, decltype(te)><<<blocks_for(L.nptiles, G.s1[k]), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par,
+                                                                                step);
+    });
     if (tm) tm->end(0, s);
   }
   if (L.nwalls) {
@@ -1976,29 +2020,21 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
   launch_phys<TS, TC, FM, false, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
-  k_fin<TS, TP, TE, METHOD><<<blocks_for(L.nptiles, g2), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles, par, step);
+  per_eq_kind<TS>(L, [&](auto te, int k) {
+    k_fin<TS, TP, decltype(te), METHOD><<<blocks_for(L.nptiles, g2[k]), kNT, 0, s>>>(L.insts, L.ptiles, L.nptiles,
+                                                                                     par, step);
+  });
   if (tm) tm->end(3, s);
-}
-
-template <typename TS, typename TC, typename TE, typename Sh>
-static void launch_stepper(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
-  switch (L.rk_method) {
-    case 0: launch_one_stage<TS, TC, TE, 0, Sh>(L, s, par, tm); break;
-    case 2: launch_one_stage<TS, TC, TE, 2, Sh>(L, s, par, tm); break;
-    case 3: launch_one_stage<TS, TC, TE, 3, Sh>(L, s, par, tm); break;
-    default: launch_rk3<TS, TC, TE, Sh>(L, s, par, tm); break;
-  }
 }
 
 template <typename TS, typename TC, typename Sh>
 static void launch_step_shape(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
-  if constexpr (sizeof(TS) == 4) {
-    if (L.eq32) {
-      launch_stepper<TS, TC, float, Sh>(L, s, par, tm);
-      return;
-    }
+  switch (L.rk_method) {
+    case 0: launch_one_stage<TS, TC, 0, Sh>(L, s, par, tm); break;
+    case 2: launch_one_stage<TS, TC, 2, Sh>(L, s, par, tm); break;
+    case 3: launch_one_stage<TS, TC, 3, Sh>(L, s, par, tm); break;
+    default: launch_rk3<TS, TC, Sh>(L, s, par, tm); break;
   }
-  launch_stepper<TS, TC, double, Sh>(L, s, par, tm);
 }
 
 // Advances the step counter of a launch sequence: the kernels of step k read

@@ -28,8 +28,8 @@ struct StepLaunch {
                                     // some instance tracks its residual; null otherwise
   int ninst = 0;                    // instances in the table (for the residual tracking)
   int nphys = 0;                    // tiles flagged kTilePhysical in `tiles` (k_phys runs when > 0)
-  int eq32 = 0;                     // FP32 storage: every instance starts near its reference
-                                    // equilibrium, equilibria in FP32 (deviation form)
+  int eq_kinds = 1;                 // equilibrium precisions among the instances: bit 0 FP64,
+                                    // bit 1 FP32 (deviation form; FP32 storage only)
 };
 
 // Flux tiles of a list flagged kTilePhysical.
@@ -39,11 +39,21 @@ inline int count_physical(const std::vector<TileRef>& tiles) {
   return n;
 }
 
+// Equilibrium precisions among a group's instances (StepLaunch::eq_kinds):
+// FP64 storage always FP64; FP32 storage per instance.
+template <typename It, typename Get>
+inline int eq_kinds_of(int fp64, It begin, It end, Get solver_eq32) {
+  if (fp64) return 1;
+  int k = 0;
+  for (It it = begin; it != end; ++it) k |= solver_eq32(*it) ? 2 : 1;
+  return k ? k : 1;
+}
+
 inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
   return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
          a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
          a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape && a.res_partial == b.res_partial && a.ninst == b.ninst &&
-         a.nphys == b.nphys && a.eq32 == b.eq32;
+         a.nphys == b.nphys && a.eq_kinds == b.eq_kinds;
 }
 
 // Optional per-kernel timing with HIP events recorded on the launch stream

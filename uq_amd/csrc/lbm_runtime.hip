@@ -406,6 +406,7 @@ static void fill_inst(const Solver& s, InstDev& d) {
       d.refsum[3] += s.ref[kQ + i];
     }
   d.i2c = 0.5 / s.c;
+  d.eq32 = s.eq32 ? 1 : 0;
   if (s.precision != LBM_PREC_FP64) {
     // base polynomials of the reference: f per unit density, h per unit rho R Tc
     static constexpr int kBase[4] = {0, 1, 5, 9};
@@ -857,7 +858,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   L.nphys = s.nphys;
-  L.eq32 = s.eq32 ? 1 : 0;
+  L.eq_kinds = s.precision == LBM_PREC_FP64 ? 1 : (s.eq32 ? 2 : 1);
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, s.graphs.get(), s.fails_dev, true);
   const int st2 = upload_inst(s);
@@ -925,7 +926,7 @@ extern "C" int lbm_step_noalloc(lbm_solver* h, int32_t nsteps, int32_t want_resi
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   L.nphys = s.nphys;
-  L.eq32 = s.eq32 ? 1 : 0;
+  L.eq_kinds = s.precision == LBM_PREC_FP64 ? 1 : (s.eq32 ? 2 : 1);
   prepare_launch(s.precision == LBM_PREC_FP64);
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, nullptr, s.fails_dev, false);
@@ -1221,8 +1222,8 @@ int uqlbm::batch_set_shape(Batch& b, int shape) {
   HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   b.ntiles = (int)tiles.size();
   b.nphys = count_physical(tiles);
-  b.eq32 = 1;
-  for (Solver* m : b.members) b.eq32 &= m->eq32 ? 1 : 0;
+  b.eq_kinds = eq_kinds_of(b.precision == LBM_PREC_FP64, b.members.begin(), b.members.end(),
+                           [](const Solver* m) { return m->eq32; });
   b.flux_shape = shape;
   return LBM_OK;
 }
@@ -1272,7 +1273,7 @@ int uqlbm::batch_step(Batch& b, int nsteps) {
   StepLaunch L{b.inst_dev,    b.tiles_dev, b.ntiles,    b.walls_dev, b.nwalls,
                b.ptiles_dev, b.nptiles,   b.rk_method, b.step_ctr,  b.flux_shape};
   L.nphys = b.nphys;
-  L.eq32 = b.eq32;
+  L.eq_kinds = b.eq_kinds;
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get(), b.fails_dev, true);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

@@ -117,7 +117,7 @@ struct Batch {
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   int nphys = 0;  // flux tiles flagged kTilePhysical
-  int eq32 = 0;   // every member near its reference equilibrium (FP32 storage)
+  int eq_kinds = 1;  // equilibrium precisions among the members (StepLaunch::eq_kinds)
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -138,7 +138,7 @@ struct Batch {
     nwalls = o.nwalls;
     nptiles = o.nptiles;
     nphys = o.nphys;
-    eq32 = o.eq32;
+    eq_kinds = o.eq_kinds;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;

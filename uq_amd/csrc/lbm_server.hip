@@ -68,7 +68,7 @@ struct Server {
   size_t tiles_cap = 0, walls_cap = 0, ptiles_cap = 0;
   int ntiles = 0, nwalls = 0, nptiles = 0;
   int nphys = 0;  // flux tiles flagged kTilePhysical
-  int eq32 = 0;   // every active request near its reference equilibrium (FP32 storage)
+  int eq_kinds = 1;  // equilibrium precisions among the active requests (StepLaunch::eq_kinds)
   bool lists_dirty = true;
   int* step_ctr = nullptr;
   int* fails_dev = nullptr;  // failure steps of the active requests, gathered for one host copy
@@ -149,8 +149,8 @@ static int rebuild_lists(Server& sv) {
   SV_TRY(hipStreamSynchronize(sv.stream));
   sv.ntiles = (int)tiles.size();
   sv.nphys = count_physical(tiles);
-  sv.eq32 = 1;
-  for (auto& a : sv.active) sv.eq32 &= a.solver->eq32 ? 1 : 0;
+  sv.eq_kinds = eq_kinds_of(sv.precision == LBM_PREC_FP64, sv.active.begin(), sv.active.end(),
+                            [](const ActiveReq& a) { return a.solver->eq32; });
   sv.nwalls = (int)walls.size();
   sv.nptiles = (int)ptiles.size();
   sv.lists_dirty = false;
@@ -348,7 +348,7 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
     StepLaunch L{sv.table_dev,  sv.tiles_dev, sv.ntiles,     sv.walls_dev, sv.nwalls,
                  sv.ptiles_dev, sv.nptiles,   sv.rk_method, sv.step_ctr,  sv.flux_shape};
     L.nphys = sv.nphys;
-    L.eq32 = sv.eq32;
+    L.eq_kinds = sv.eq_kinds;
     if (track) {
       L.res_partial = sv.res_partial;
       L.ninst = (int)members.size();
