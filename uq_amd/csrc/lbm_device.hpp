@@ -152,84 +152,96 @@ __host__ __device__ __forceinline__ void eq_dir(const EqIn<T>& q, T& fe, T& he) 
 // deviation instead of to the full value. Same closed forms as above
 // (reference inlineFunctions.cpp:10-61), regrouped.
 // ---------------------------------------------------------------------------
-struct EqDev {
-  float u, v, t, dt;    // velocity / sqrt(R Tc), T/Tc, T/Tc - t0
-  float tt, t2s;        // t + t0, t^2 + t t0 + t0^2
-  float rho, drho;      // density, density - rho0
-  float K, scale, dscale;  // internal degrees of freedom, rho R Tc, (rho - rho0) R Tc
+// The kernels evaluate it in FP32 (EqDev); the FP64 instance checks the
+// regrouping against the closed form on the host.
+template <typename T>
+struct EqDevT {
+  T u, v, t, dt;      // velocity / sqrt(R Tc), T/Tc, T/Tc - t0
+  T tt, t2s;          // t + t0, t^2 + t t0 + t0^2
+  T rho, drho;        // density, density - rho0
+  T K, scale, dscale;  // internal degrees of freedom, rho R Tc, (rho - rho0) R Tc
 };
+using EqDev = EqDevT<float>;
 
-__device__ __forceinline__ float pf0_uv(float u, float v, float t) {
-  const float u2 = u * u, v2 = v * v;
-  return 0.25f * (u2 * u2 - 5.f * v2 + v2 * v2 + 10.f * t * (u2 + v2) + u2 * (-5.f + 4.f * v2));
+template <typename T>
+__host__ __device__ __forceinline__ T pf0_uv(T u, T v, T t) {
+  const T u2 = u * u, v2 = v * v;
+  return T(0.25) * (u2 * u2 - T(5) * v2 + v2 * v2 + T(10) * t * (u2 + v2) + u2 * (-T(5) + T(4) * v2));
 }
-__device__ __forceinline__ float pf1_uv(float a, float b, float t) {
-  const float a2 = a * a, b2 = b * b;
-  return (1.f / 6.f) * (-a * (1.f + a) * (-4.f + a2 + 3.f * b2) - t * (6.f * a + 9.f * a2 + 3.f * b2));
+template <typename T>
+__host__ __device__ __forceinline__ T pf1_uv(T a, T b, T t) {
+  const T a2 = a * a, b2 = b * b;
+  return (T(1) / T(6)) * (-a * (T(1) + a) * (-T(4) + a2 + T(3) * b2) - t * (T(6) * a + T(9) * a2 + T(3) * b2));
 }
-__device__ __forceinline__ float pf5_uv(float a, float b, float t) {
-  return 0.25f * (t * (b + b * b) + (a + a * a) * (t + b + b * b));
+template <typename T>
+__host__ __device__ __forceinline__ T pf5_uv(T a, T b, T t) {
+  return T(0.25) * (t * (b + b * b) + (a + a * a) * (t + b + b * b));
 }
-__device__ __forceinline__ float pf9_uv(float a, float t) {
-  return (1.f / 24.f) * ((-1.f + a) * a * (1.f + a) * (2.f + a) + 6.f * t * a * (1.f + a));
+template <typename T>
+__host__ __device__ __forceinline__ T pf9_uv(T a, T t) {
+  return (T(1) / T(24)) * ((-T(1) + a) * a * (T(1) + a) * (T(2) + a) + T(6) * t * a * (T(1) + a));
 }
-__device__ __forceinline__ float ph0_uv(float K, float t, float u, float v) {
-  const float u2 = u * u, v2 = v * v, u4 = u2 * u2, v4 = v2 * v2;
-  return (3.f * t * (-40.f * u2 - 5.f * K * u2 + 20.f * u4 + K * u4 + (-5.f * (8.f + K) + 4.f * (15.f + K) * u2) * v2 +
-                     (20.f + K) * v4) +
-          30.f * (t * t) * ((9.f + K) * (u2 + v2)) +
-          3.f * (u2 + v2) * (4.f + u4 - 5.f * v2 + v4 + u2 * (-5.f + 4.f * v2))) *
-         (1.f / 24.f);
+template <typename T>
+__host__ __device__ __forceinline__ T ph0_uv(T K, T t, T u, T v) {
+  const T u2 = u * u, v2 = v * v, u4 = u2 * u2, v4 = v2 * v2;
+  return (T(3) * t * (-T(40) * u2 - T(5) * K * u2 + T(20) * u4 + K * u4 + (-T(5) * (T(8) + K) + T(4) * (T(15) + K) * u2) * v2 +
+                     (T(20) + K) * v4) +
+          T(30) * (t * t) * ((T(9) + K) * (u2 + v2)) +
+          T(3) * (u2 + v2) * (T(4) + u4 - T(5) * v2 + v4 + u2 * (-T(5) + T(4) * v2))) *
+         (T(1) / T(24));
 }
-__device__ __forceinline__ float ph1_uv(float K, float t, float a, float b) {
-  const float a2 = a * a, a3 = a2 * a, b2 = b * b;
-  return (-a * (1.f + a) * (a2 + b2) * (-4.f + a2 + 3.f * b2) -
-          t * (a * (-4.f * (4.f + K) - 4.f * (7.f + K) * a + (14.f + K) * a2 + (19.f + K) * a3) +
-               (-4.f + 3.f * a * (10.f + K + (14.f + K) * a)) * b2 + 3.f * (b2 * b2)) -
-          (t * t) * (6.f * a * (6.f + 13.f * a) + 30.f * b2 + K * (6.f * a + 9.f * a2 + 3.f * b2))) *
-         (1.f / 12.f);
+template <typename T>
+__host__ __device__ __forceinline__ T ph1_uv(T K, T t, T a, T b) {
+  const T a2 = a * a, a3 = a2 * a, b2 = b * b;
+  return (-a * (T(1) + a) * (a2 + b2) * (-T(4) + a2 + T(3) * b2) -
+          t * (a * (-T(4) * (T(4) + K) - T(4) * (T(7) + K) * a + (T(14) + K) * a2 + (T(19) + K) * a3) +
+               (-T(4) + T(3) * a * (T(10) + K + (T(14) + K) * a)) * b2 + T(3) * (b2 * b2)) -
+          (t * t) * (T(6) * a * (T(6) + T(13) * a) + T(30) * b2 + K * (T(6) * a + T(9) * a2 + T(3) * b2))) *
+         (T(1) / T(12));
 }
-__device__ __forceinline__ float ph5_uv(float K, float t, float a, float b) {
-  const float a2 = a * a, b2 = b * b;
-  return (3.f * a * (1.f + a) * b * (1.f + b) * (a2 + b2) +
-          3.f * (t * t) * ((6.f + K) * a + (9.f + K) * a2 + b * (6.f + K + (9.f + K) * b)) +
-          3.f * t *
-              (a2 * a + a2 * a2 + b2 * b * (1.f + b) + a * b * (6.f + K + (9.f + K) * b) +
-               a2 * b * (9.f + K + (12.f + K) * b))) *
-         (1.f / 24.f);
+template <typename T>
+__host__ __device__ __forceinline__ T ph5_uv(T K, T t, T a, T b) {
+  const T a2 = a * a, b2 = b * b;
+  return (T(3) * a * (T(1) + a) * b * (T(1) + b) * (a2 + b2) +
+          T(3) * (t * t) * ((T(6) + K) * a + (T(9) + K) * a2 + b * (T(6) + K + (T(9) + K) * b)) +
+          T(3) * t *
+              (a2 * a + a2 * a2 + b2 * b * (T(1) + b) + a * b * (T(6) + K + (T(9) + K) * b) +
+               a2 * b * (T(9) + K + (T(12) + K) * b))) *
+         (T(1) / T(24));
 }
-__device__ __forceinline__ float ph9_uv(float K, float t, float a, float b) {
-  const float a2 = a * a, b2 = b * b;
-  return (t * a * (-8.f + K * (-1.f + a) * (1.f + a) * (2.f + a) + a * (-7.f + 2.f * a * (11.f + 8.f * a))) +
-          t * (-1.f + 6.f * a * (1.f + a)) * b2 + (-1.f + a) * a * (1.f + a) * (2.f + a) * (a2 + b2) +
-          (t * t) * (36.f * a + 51.f * a2 + 6.f * K * a * (1.f + a) + 3.f * b2)) *
-         (1.f / 48.f);
+template <typename T>
+__host__ __device__ __forceinline__ T ph9_uv(T K, T t, T a, T b) {
+  const T a2 = a * a, b2 = b * b;
+  return (t * a * (-T(8) + K * (-T(1) + a) * (T(1) + a) * (T(2) + a) + a * (-T(7) + T(2) * a * (T(11) + T(8) * a))) +
+          t * (-T(1) + T(6) * a * (T(1) + a)) * b2 + (-T(1) + a) * a * (T(1) + a) * (T(2) + a) * (a2 + b2) +
+          (t * t) * (T(36) * a + T(51) * a2 + T(6) * K * a * (T(1) + a) + T(3) * b2)) *
+         (T(1) / T(48));
 }
 
 // f and h equilibrium deviations of direction I; eqref holds the base
 // polynomials at rest at t0 (f per unit density, h per unit scale).
-template <int I>
-__device__ __forceinline__ void eq_dev(const EqDev& q, const float* eqref, float& fe, float& he) {
+template <int I, typename T>
+__host__ __device__ __forceinline__ void eq_dev(const EqDevT<T>& q, const T* eqref, T& fe, T& he) {
   constexpr int base = eq_base(I);
-  const float a = float(eq_sa(I)) * (eq_swap(I) ? q.v : q.u);
-  const float b = float(eq_sb(I)) * (eq_swap(I) ? q.u : q.v);
-  const float K = q.K, t = q.t, dt = q.dt, tt = q.tt, t2s = q.t2s;
-  const float c3 = 16.f + 3.f * K;
+  const T a = T(eq_sa(I)) * (eq_swap(I) ? q.v : q.u);
+  const T b = T(eq_sb(I)) * (eq_swap(I) ? q.u : q.v);
+  const T K = q.K, t = q.t, dt = q.dt, tt = q.tt, t2s = q.t2s;
+  const T c3 = T(16) + T(3) * K;
   if constexpr (base == 0) {
-    fe = q.rho * (pf0_uv(q.u, q.v, t) + 2.5f * dt * (tt - 1.f)) + q.drho * eqref[0];
+    fe = q.rho * (pf0_uv(q.u, q.v, t) + T(2.5) * dt * (tt - T(1))) + q.drho * eqref[0];
     he = q.scale * (ph0_uv(K, t, q.u, q.v) +
-                    dt * (10.f * c3 * t2s + 3.f * (8.f + 4.f * K) - 30.f * (4.f + K) * tt) * (1.f / 24.f)) +
+                    dt * (T(10) * c3 * t2s + T(3) * (T(8) + T(4) * K) - T(30) * (T(4) + K) * tt) * (T(1) / T(24))) +
          q.dscale * eqref[4];
   } else if constexpr (base == 1) {
-    fe = q.rho * (pf1_uv(a, b, t) + dt * (-6.f * tt + 4.f) * (1.f / 6.f)) + q.drho * eqref[1];
-    he = q.scale * (ph1_uv(K, t, a, b) + dt * (-2.f * c3 * t2s + (16.f + 4.f * K) * tt) * (1.f / 12.f)) +
+    fe = q.rho * (pf1_uv(a, b, t) + dt * (-T(6) * tt + T(4)) * (T(1) / T(6))) + q.drho * eqref[1];
+    he = q.scale * (ph1_uv(K, t, a, b) + dt * (-T(2) * c3 * t2s + (T(16) + T(4) * K) * tt) * (T(1) / T(12))) +
          q.dscale * eqref[5];
   } else if constexpr (base == 5) {
-    fe = q.rho * (pf5_uv(a, b, t) + 0.25f * dt * tt) + q.drho * eqref[2];
-    he = q.scale * (ph5_uv(K, t, a, b) + dt * c3 * t2s * (1.f / 24.f)) + q.dscale * eqref[6];
+    fe = q.rho * (pf5_uv(a, b, t) + T(0.25) * dt * tt) + q.drho * eqref[2];
+    he = q.scale * (ph5_uv(K, t, a, b) + dt * c3 * t2s * (T(1) / T(24))) + q.dscale * eqref[6];
   } else {
-    fe = q.rho * (pf9_uv(a, t) + dt * (3.f * tt - 1.f) * (1.f / 24.f)) + q.drho * eqref[3];
-    he = q.scale * (ph9_uv(K, t, a, b) + dt * (c3 * t2s - (4.f + K) * tt) * (1.f / 48.f)) + q.dscale * eqref[7];
+    fe = q.rho * (pf9_uv(a, t) + dt * (T(3) * tt - T(1)) * (T(1) / T(24))) + q.drho * eqref[3];
+    he = q.scale * (ph9_uv(K, t, a, b) + dt * (c3 * t2s - (T(4) + K) * tt) * (T(1) / T(48))) + q.dscale * eqref[7];
   }
 }
 
@@ -267,25 +279,26 @@ __device__ __forceinline__ void finish_macro(const InstDev& in, double rho, doub
   m.ithf = m.itf * in.Pr_m1;
 }
 
-__device__ __forceinline__ EqDev eq_dev_inputs(const InstDev& in, const Macro& m) {
-  EqDev q;
+template <typename T = float>
+__host__ __device__ __forceinline__ EqDevT<T> eq_dev_inputs(const InstDev& in, const Macro& m) {
+  EqDevT<T> q;
   const double t = m.T * in.iTc;
-  q.u = float(m.ux * in.isRTc);
-  q.v = float(m.uy * in.isRTc);
-  q.t = float(t);
-  q.dt = float(t - in.t0);
-  q.tt = float(t + in.t0);
-  q.t2s = float(t * t + t * in.t0 + in.t0 * in.t0);
-  q.rho = float(m.rho);
-  q.drho = float(m.rho - in.rho0);
-  q.K = float(in.K);
-  q.scale = float(m.rho * in.RTc);
-  q.dscale = float((m.rho - in.rho0) * in.RTc);
+  q.u = T(m.ux * in.isRTc);
+  q.v = T(m.uy * in.isRTc);
+  q.t = T(t);
+  q.dt = T(t - in.t0);
+  q.tt = T(t + in.t0);
+  q.t2s = T(t * t + t * in.t0 + in.t0 * in.t0);
+  q.rho = T(m.rho);
+  q.drho = T(m.rho - in.rho0);
+  q.K = T(in.K);
+  q.scale = T(m.rho * in.RTc);
+  q.dscale = T((m.rho - in.rho0) * in.RTc);
   return q;
 }
 
 template <typename T>
-__device__ __forceinline__ EqIn<T> eq_inputs(const InstDev& in, const Macro& m) {
+__host__ __device__ __forceinline__ EqIn<T> eq_inputs(const InstDev& in, const Macro& m) {
   EqIn<T> q;
   q.rho = T(m.rho);
   q.u = T(m.ux * in.isRTc);
