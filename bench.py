@@ -363,6 +363,24 @@ def _cpu_job(job):
     return c.nx, c.cells * done, sec
 
 
+def _cpu_large(conn, n: int, steps: int, cfl: float) -> None:
+    """One host process: the oracle (one core) on the n^2 Riemann deck for
+    `steps` steps, the CPU cost of one large instance; the result goes back
+    through `conn`."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from pyoracle import Oracle
+        o = Oracle(cases.riemann(n, steps=steps, cfl=cfl))
+        o.step(steps)
+        sec = o.step_seconds()
+        o.close()
+        conn.send({"n": n, "steps": steps, "seconds": round(sec, 2), "mlups": round(n * n * steps / sec / 1e6, 4),
+                   "cores": 1, "kind": "port"})
+    except Exception as e:  # reported, not raised: the GPU legs' results stand
+        conn.send({"error": str(e)})
+    conn.close()
+
+
 def cpu_leg(members, seconds: float, cores: int) -> dict:
     """The CPU oracle (plain C FP64, serial, -O2; the reference's numerics)
     on this host: one process per core, each on its own ensemble member
@@ -414,6 +432,8 @@ def main() -> int:
     ap.add_argument("--serve-sustained-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0, help="host processes of the CPU baseline (0: all, <= 16)")
+    ap.add_argument("--cpu-riemann-steps", type=int, default=1,
+                    help="steps of the one-core CPU oracle on the Riemann deck (0: off); runs beside the later legs")
     ap.add_argument("--quick", action="store_true", help="small smoke-sized run")
     ap.add_argument("--no-extras", action="store_true", help="headline leg only")
     ap.add_argument("--legs", default="all",
@@ -525,6 +545,19 @@ def main() -> int:
         result["couette64"] = couette_leg()
         log(f"couette64: {result['couette64']}")
 
+    # the one-core CPU oracle on the large Riemann deck takes about two
+    # minutes per step at 4096^2: it runs in a process of its own beside the
+    # remaining GPU legs (which hold one host core) and is collected before
+    # the CPU leg's ensemble pool starts
+    large = None
+    if "cpu" in legs and args.cpu_riemann_steps > 0 and not args.quick:
+        ctx = mp.get_context("spawn")
+        rd, wr = ctx.Pipe(duplex=False)
+        proc = ctx.Process(target=_cpu_large, args=(wr, args.riemann_n, args.cpu_riemann_steps, args.riemann_cfl))
+        proc.start()
+        wr.close()
+        large = (proc, rd)
+
     if "plate" in legs and args.plate_steps > 0:
         result["plate512x256"] = plate_leg(args.plate_steps)
         log(f"plate: {result['plate512x256']}")
@@ -555,7 +588,17 @@ def main() -> int:
         # --- CPU baseline: the oracle on this host's cores -------------------
         try:
             cores = args.cpu_cores if args.cpu_cores > 0 else min(16, os.cpu_count() or 1)
+            big = None
+            if large is not None:  # collected first: the ensemble pool below then has the host to itself
+                proc, rd = large
+                big = rd.recv() if rd.poll(900) else {"error": "did not finish within 900 s"}
+                proc.join(10)
+                if proc.is_alive():
+                    proc.terminate()
+                log(f"cpu riemann: {big}")
             result["cpu_baseline"] = cpu_leg(members, args.cpu_seconds, cores)
+            if big is not None:
+                result["cpu_baseline"]["riemann_single_core"] = big
             log(f"cpu: {result['cpu_baseline']}")
         except Exception as e:  # the oracle library is built by __graft_entry__.build()
             result["cpu_baseline"] = {"value": None, "error": str(e)}

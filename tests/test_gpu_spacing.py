@@ -134,12 +134,23 @@ def test_spacing_set_after_init_on_a_batch_member(gpu_lib):
 
 
 def test_mixed_precision_on_a_stretched_grid(gpu_lib):
+    """Mixed precision's bar is on the flow fields (rho, u, T) against the
+    FP64 path, not on the distributions, whose relative L2 is dominated by
+    the rest-state populations."""
     c = cases.stretched(cases.shear_wave(48, u0=0.15, steps=30), amp=0.3)
-    st64, f64, _ = _gpu(gpu_lib, c, 30)
-    st32, f32, _ = _gpu(gpu_lib, c, 30, precision="mixed")
-    e = rel_l2(f32, f64)
-    print(f"[spacing] mixed vs fp64: {e:.2e}", flush=True)
-    assert st64 == 0 and st32 == 0 and e <= 1e-6
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = gpu_lib.Solver(c, prec)
+        try:
+            assert s.step(30) == 0
+            out[prec] = s.get_macro()
+        finally:
+            s.close()
+    m32, m64 = out["mixed"], out["fp64"]
+    e = {k: rel_l2(m32[k], m64[k]) for k in ("rho", "T")}
+    e["u"] = rel_l2(np.stack([m32["ux"], m32["uy"]]), np.stack([m64["ux"], m64["uy"]]))
+    print(f"[spacing] mixed vs fp64 over 30 steps: {e}", flush=True)
+    assert all(v <= 1e-6 for v in e.values()), e
 
 
 @pytest.mark.parametrize("deck", ["obstacle", "couette"])
