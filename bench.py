@@ -69,19 +69,20 @@ def kernel_bytes_per_lup(es: int) -> dict:
 
 
 def measured_traffic(kernel: str, workload: str, shape: int):
-    """HBM bytes per launch of `kernel` on `workload` at flux tile `shape`,
-    from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes committed under
-    profiles/ (copy-calibrated), or None when no pass at that shape exists."""
+    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes committed under profiles/
+    (copy-calibrated): the pointwise kernels' records hold for every flux
+    tile shape, the flux kernels' are keyed `kernel@shape`. None when no
+    pass covers the kernel at that shape."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
             tab = json.load(fh)
     except (OSError, ValueError):
         return None
-    rec = tab.get(workload, {}).get(kernel)
-    if rec is None or rec.get("flux_shape") != shape:
-        return None
-    return rec.get("bytes_per_launch")
+    key = f"{kernel}@{shape}" if kernel.startswith("flux") else kernel
+    rec = tab.get(workload, {}).get(key)
+    return None if rec is None else rec.get("bytes_per_launch")
 
 
 def batched_leg(members, precision: str, steps: int, warmup: int, timing_steps: int, tune_steps: int):
@@ -191,7 +192,7 @@ def plain_leg(members, steps: int, shapes) -> dict:
 
 
 def couette_leg() -> dict:
-    """Config 1: the 64x64 Couette deck (u_wall 0.1, tau 0.05, Pr 0.71,
+    """The 64x64 Couette deck (u_wall 0.1, tau 0.05, Pr 0.71,
     WENO5, RK3) over its 2000 steps against the oracle fixture of the same
     run, then continued to steady state against the slip Couette profile."""
     c = cases.couette(64, steps=2000)
@@ -228,7 +229,7 @@ def couette_leg() -> dict:
 
 
 def plate_leg(steps: int) -> dict:
-    """Config 2: the 512x256 Mach-2 flat plate (NND, RK3, inflow column,
+    """The 512x256 Mach-2 flat plate (NND, RK3, inflow column,
     free-stream row, symmetry plane then isothermal plate) over its steps;
     property checks: no failure, finite fields, a boundary layer (the
     velocity next to the plate below half the free stream)."""
@@ -248,7 +249,7 @@ def plate_leg(steps: int) -> dict:
 
 
 def riemann_leg(n: int, steps: int, time_steps: int, shape, cfl: float) -> dict:
-    """Config 5: the four-quadrant Riemann problem at n^2 (the reference's
+    """The four-quadrant Riemann problem at n^2 (the reference's
     Lax-Liu set-up, physical air viscosity) in FP64 and mixed precision:
     `time_steps` steps of each timed, then both continued side by side to
     `steps`, with the relative L2 of density and temperature between the two
