@@ -56,16 +56,16 @@ def log(msg: str) -> None:
 def kernel_bytes_per_lup(es: int) -> dict:
     """Compulsory HBM traffic of each kernel per lattice update, for values
     of `es` bytes. FP64 storage: s12 reads the state and writes f2 (52
-    values); flux2 reads f2 and writes the 24 net-flux planes and their 4
-    FP64 moments; s3 reads the state, the net fluxes and their moments and
-    writes f3 and G (102 values + 4 FP64); flux3 reads f3 and G and writes
-    the state (78). FP32 storage adds the 4 FP64 conserved-moment planes of
-    the state: read by s12 and s3, written by s3 (those of G) and read and
-    written by flux3. Halo re-reads, the cell-type map and the wall-closure
-    kernels are excluded."""
+    values); flux2 reads f2 and writes the 24 net-flux planes (50); s3
+    reads the state and the net fluxes (summing the net fluxes' moments in
+    a first pass whose lines the second pass finds in L2) and writes f3 and
+    G (102); flux3 reads f3 and G and writes the state (78). FP32 storage
+    adds the 4 FP64 conserved-moment planes of the state: read by s12 and
+    s3, written by s3 (those of G) and read and written by flux3. Halo
+    re-reads, the cell-type map and the wall-closure kernels are
+    excluded."""
     mom = 32 if es == 4 else 0
-    return {"s12": 52 * es + mom, "flux2": 50 * es + 32, "s3": 102 * es + 32 + 2 * mom,
-            "flux3": 78 * es + 2 * mom}
+    return {"s12": 52 * es + mom, "flux2": 50 * es, "s3": 102 * es + 2 * mom, "flux3": 78 * es + 2 * mom}
 
 
 def measured_traffic(kernel: str, workload: str, shape: int):

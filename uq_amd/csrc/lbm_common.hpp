@@ -48,8 +48,6 @@ struct InstDev {
   const uint8_t* ctype;
   double* alpha;       // wall closure factor, [ny][pitch]
   int* fail;           // per-instance: first failing step (INT_MAX when none)
-  double* dmom;        // 4 FP64 planes: moments of the stage-2 net fluxes of a
-                       // fluid cell (sum dF, sum dF e_x c, sum dF e_y c, sum dH)
   double* mom[2];      // FP32 storage: conserved moments of state[k] (4 FP64
                        // planes each); null for FP64 storage
   double ref[26];      // FP32 storage: reference equilibrium f0..f12, h0..h12

@@ -411,11 +411,18 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
   const double* s = cs.s;
   // stage-1 moments, and the stage-3 moments: those of f - dF(f2)
   // (macroProp3, cpuLBM.cpp:973-1032), formed as moments of f minus the
-  // moments of dF that the stage-2 flux kernel summed for this cell, so that
-  // every direction can be finished in one pass
-  const double drho = __builtin_nontemporal_load(&in.dmom[o]), drux = __builtin_nontemporal_load(&in.dmom[P + o]),
-               druy = __builtin_nontemporal_load(&in.dmom[2 * P + o]),
-               dsh = __builtin_nontemporal_load(&in.dmom[3 * P + o]);
+  // moments of the stored dF, summed here in a first pass over the
+  // directions (the loads leave the lines in L2 for the second pass), so
+  // that every direction can then be finished in one pass
+  double drho = 0, drux = 0, druy = 0, dsh = 0;
+#pragma unroll
+  for (int I = 1; I < kQ; I++) {
+    const double dFs = double(dfb[P * (I - 1) + o]);
+    drho += dFs;
+    drux += dFs * (lat_x(I) * in.c);
+    druy += dFs * (lat_y(I) * in.c);
+    dsh += double(dfb[P * (12 + I - 1) + o]);
+  }
   Macro m1, m3;
   finish_macro(in, s[0], s[1], s[2], s[3], m1);
   finish_macro(in, s[0] - drho, s[1] - drux, s[2] - druy, s[3] - dsh, m3);
@@ -1032,8 +1039,9 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
 //   MODE 0: source f2 (f1 for RK1); dF, dH -> the dF planes.
 //   MODE 1: source f3; new state (holding G) -= dF/2, dH/2.
 //   MODE 2: source the current state (EULER, EULER_EQ); dF, dH -> dF planes.
-// MOM (with MODE 0 of RK3): also the moments of each fluid cell's stored
-// dF, dH over the directions -> in.dmom.
+// MOM (MODE 1, FP32 storage): also the new state's conserved moments,
+// those of G minus half those of each fluid cell's dF(f3) over the
+// directions -> the moment planes of the new state buffer.
 //
 // FP32 storage: directions alternate between two staging buffers filled by
 // direct-to-LDS loads; while the threads compute direction I's faces from
@@ -1402,21 +1410,18 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
     if constexpr (!MOM) fetch_slots<TS>(PV, L, W.src, W.P, 1);
     for (int I = 1; I < kQ; I++) flux_dir<TS, TC, MODE, MOM, false>(S, R, in, W, L, PV, VA, VA, I, I + 1 < kQ);
   }
-  if (MOM) {
-    // MODE 0: the moments of dF(f2) for stage 3. MODE 1 (FP32 storage): the
-    // new state's conserved moments, those of G (written by stage 3) minus
-    // half those of dF(f3).
-    double* mo = MODE == 1 ? in.mom[1 ^ in.parity0 ^ par] : in.dmom;
+  if constexpr (MOM) {
+    // the new state's conserved moments: those of G (written by stage 3)
+    // minus half those of dF(f3)
+    static_assert(MODE == 1, "moment sums belong to the stage-3 flux of FP32 storage");
+    double* mo = in.mom[1 ^ in.parity0 ^ par];
 #pragma unroll
     for (int q = 0; q < Sh::Cells; q++) {
       if (!W.fluid[q]) continue;
       const int ci = t + q * Sh::NT;
       const long long o = W.oo[q];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (MODE == 1) mo[k * W.P + o] -= 0.5 * S.mom[k][ci];
-        else mo[k * W.P + o] = S.mom[k][ci];
-      }
+      for (int k = 0; k < 4; k++) mo[k * W.P + o] -= 0.5 * S.mom[k][ci];
     }
   }
 }
@@ -1442,9 +1447,8 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
 // whose net fluxes the logical addressing of the staged tiles would get
 // wrong: each thread takes an own fluid cell and forms its twelve net
 // fluxes with the reference's addressing (phys_net), then writes what the
-// flux kernel writes for the cell (MODE 0 and 2: the dF planes, with MOM
-// the moment sums; MODE 1: the new state G - dF/2, with MOM the moment
-// planes' update). Launched after the flux kernel, only for groups with
+// flux kernel writes for the cell (MODE 0 and 2: the dF planes; MODE 1: the
+// new state G - dF/2, with MOM the moment planes' update). Launched after the flux kernel, only for groups with
 // such tiles; the flux kernel skips them.
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 __global__ __launch_bounds__(256) void k_phys(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
@@ -1487,27 +1491,22 @@ __global__ __launch_bounds__(256) void k_phys(const InstDev* insts, const TileRe
         m[2] += vf * ey;
         m[3] += vh;
       }
-      if (MOM) {
-        double* mo = MODE == 1 ? in.mom[nxt] : in.dmom;
-        for (int k = 0; k < 4; k++) {
-          if (MODE == 1) mo[k * P + o] -= 0.5 * m[k];
-          else mo[k * P + o] = m[k];
-        }
+      if constexpr (MOM) {
+        static_assert(MODE == 1, "moment sums belong to the stage-3 flux of FP32 storage");
+        for (int k = 0; k < 4; k++) in.mom[nxt][k * P + o] -= 0.5 * m[k];
       }
     }
   }
 }
 
 // FP32 storage: capped at 64 registers for 8 waves per SIMD (4 blocks per
-// CU) except with the moment sums (3 spilled slots; mixed-ensemble stage-3
-// flux 3.35 -> 3.18 ms, while the moment-summing stage measured 2 % slower
-// capped and keeps the compiler's allocation). FP64: capped at 80 registers
-// for 6 waves per SIMD.
+// CU; with the stage-3 moment sums 3 spilled slots, mixed-ensemble stage-3
+// flux 3.35 -> 3.18 ms). FP64: capped at 80 registers for 6 waves per SIMD.
 // (The staging buffers are separate LDS variables so that the compiler sees
 // that reads of one do not wait for the loads into the other; the second
 // is dropped where it is unused.)
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
-__global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(MOM && MODE == 0 ? 2 : 8))) void k_flux(
+__global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(8))) void k_flux(
     const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, TC, MOM> S;
@@ -1892,7 +1891,7 @@ constexpr bool kMom3 = sizeof(TS) == 4;
 template <typename TS, typename TC, typename Sh>
 struct StepGrids {
   int s12[2] = {0, 0}, s3[2] = {0, 0}, s1[2] = {0, 0}, fin0[2] = {0, 0}, fin2[2] = {0, 0}, fin3[2] = {0, 0};
-  int flux2, flux3, flux0, fluxe;
+  int flux3, flux0, fluxe;  // flux0 also serves the stage-2 flux of RK3
   template <typename TE>
   void pointwise(int k) {
     s12[k] = grid_for((const void*)k_s12<TS, TP, TE>, kNT);
@@ -1906,7 +1905,6 @@ struct StepGrids {
     StepGrids g;
     g.template pointwise<double>(0);
     if constexpr (sizeof(TS) == 4) g.template pointwise<float>(1);
-    g.flux2 = grid_for((const void*)FluxKernel<TS, TC, 0, true, Sh>::fn, Sh::NT);
     g.flux3 = grid_for((const void*)FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn, Sh::NT);
     g.flux0 = grid_for((const void*)FluxKernel<TS, TC, 0, false, Sh>::fn, Sh::NT);
     g.fluxe = grid_for((const void*)FluxKernel<TS, TC, 2, false, Sh>::fn, Sh::NT);
@@ -1957,7 +1955,7 @@ template <typename TS, typename TC, typename Sh>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
   const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
-  const int g1 = G.flux2, g3 = G.flux3;
+  const int g1 = G.flux0, g3 = G.flux3;
   const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
@@ -1972,8 +1970,8 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  FluxKernel<TS, TC, 0, true, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
-  launch_phys<TS, TC, 0, true, Sh>(L, s, par);
+  FluxKernel<TS, TC, 0, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
+  launch_phys<TS, TC, 0, false, Sh>(L, s, par);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   per_eq_kind<TS>(L, [&](auto te, int k) {

@@ -393,7 +393,6 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.ctype = s.ctype;
   d.alpha = s.alpha;
   d.fail = s.fail;
-  d.dmom = s.macro_dev;  // planes 0-3 of the output buffer, free during a step
   d.mom[0] = s.mom[0];
   d.mom[1] = s.mom[1];
   for (int k = 0; k < kPlanes; k++) d.ref[k] = s.precision == LBM_PREC_FP64 ? 0.0 : s.ref[k];
