@@ -275,6 +275,21 @@ int lbm_get_timing(lbm_timing* out);
 int lbm_set_graphs(int32_t enable);
 
 /*
+ * Flux stages of instances whose cells all take the logical stencil
+ * addressing can run on the strip kernels (one wavefront per 58-column,
+ * 32-row strip, marched row by row) instead of the flux tile kernels, except
+ * the moment-summing stage-3 flux of LBM_PREC_MIXED. mode 0: every flux
+ * stage on the tile kernels; 1 (default): strips for launch groups with at
+ * least 2048 strip items (enough to fill the GPU; e.g. a 4096^2 instance or
+ * the 256-member ensemble), tiles for smaller groups; 2: strips wherever an
+ * instance can take them. Results agree to FP64 (FP32) rounding, not
+ * bitwise: the two order a few operations differently. Applies to step
+ * calls made afterwards; LBM_ERR_ARG for another mode. No counterpart in
+ * the reference.
+ */
+int lbm_set_strips(int32_t mode);
+
+/*
  * Continuous-batching server. Requests are queued host-side; a request is
  * admitted (device state allocated and initialised) when a slot is free and
  * its device footprint fits within hbm_budget_bytes. Each tick advances

@@ -90,11 +90,19 @@ struct InstDev {
   double* res_hist;
   int res_last;
   int ptile0, nptiles;
+  // the group's strip list holds this instance: the tile flux kernels skip
+  // its tiles, except the moment-summing stage-3 flux of FP32 storage
+  int strips;
 };
 
 constexpr int kResHist = 20;  // residual history slots per tracked instance
 
 // One tile of one instance: the unit of work of a block.
+// Strip work items of the strip flux kernels (TileRef with tx = strip index,
+// ty = row block): kStripW own columns (with a 3-column halo on each side,
+// 64 lanes) by up to kStripH rows.
+constexpr int kStripW = 58, kStripH = 32;
+
 struct TileRef {
   int inst;
   int tx, ty;

@@ -1337,7 +1337,7 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
     if (MODE != 1) {
       W.dst[P * (I - 1) + o] = TS(dF);
       W.dst[P * (12 + I - 1) + o] = TS(dH);
-      if (MOM) {
+      if constexpr (MOM) {
         const double dFs = double(TS(dF));
         S.mom[0][ci] += dFs;
         S.mom[1][ci] += dFs * (ex * c);
@@ -1347,7 +1347,7 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
     } else {
       W.dst[P * I + o] = TS(TC(oldf[q]) - TC(0.5) * dF);
       W.dst[P * (kQ + I) + o] = TS(TC(oldh[q]) - TC(0.5) * dH);
-      if (MOM) {  // conserved moments of dF(f3), for the FP64 moment planes
+      if constexpr (MOM) {  // conserved moments of dF(f3), for the FP64 moment planes
         const double dFs = double(dF);
         S.mom[0][ci] += dFs;
         S.mom[1][ci] += dFs * (ex * c);
@@ -1435,6 +1435,7 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
     const InstDev& in = insts[tr.inst];
     if (frozen_by_failure(in, step)) continue;
     if (tr.flags & kTilePhysical) continue;  // k_phys forms this tile's net fluxes
+    if (!MOM && in.strips) continue;         // the strip kernels form this instance's net fluxes
     __syncthreads();
     setup_region(R, in, tr.tx * Sh::TX, tr.ty * Sh::TY);
     __syncthreads();
@@ -1524,6 +1525,8 @@ __global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(6))) voi
   __shared__ StageBuf<Sh, double> VB;
   flux_tiles<double, double, MODE, MOM>(insts, tiles, ntiles, par, *stepp, R, S, VA, VB);
 }
+
+#include "lbm_strip.hpp"
 
 // The flux kernel of a storage/compute type pair and tile shape.
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
@@ -1916,6 +1919,58 @@ struct StepGrids {
   }
 };
 
+// The physical-addressing net fluxes of the flagged tiles, after the flux
+// kernel of the same MODE.
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+static void launch_phys(const StepLaunch& L, hipStream_t s, int par) {
+  if (L.nphys > 0)
+    k_phys<TS, TC, MODE, MOM, Sh><<<L.ntiles < 1024 ? L.ntiles : 1024, 256, 0, s>>>(L.insts, L.tiles, L.ntiles, par,
+                                                                                  L.step_ctr);
+}
+
+// Persistent grids of the strip kernels of one storage/compute type and
+// MODE (without and with solids).
+template <typename TS, typename TC, int MODE>
+struct StripGrids {
+  int fast = 0, solid = 0;
+  static const StripGrids& get() {
+    static const StripGrids g = [] {
+      StripGrids r;
+      r.fast = grid_for((const void*)k_strip<TS, TC, MODE, false>, 256);
+      r.solid = grid_for((const void*)k_strip<TS, TC, MODE, true>, 256);
+      return r;
+    }();
+    return g;
+  }
+};
+
+// One flux stage of MODE: the strip kernels for the instances on strips and
+// the tile kernel for the others (every instance for the moment-summing
+// stage-3 flux of FP32 storage, which has no strip form), then the
+// physical-addressing net fluxes of flagged tiles.
+template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
+static void launch_flux(const StepLaunch& L, hipStream_t s, int par, int grid_tiles) {
+  const int* step = L.step_ctr;
+  bool tiles = true;
+  if constexpr (!MOM) {
+    if (L.nstrips > 0) {
+      const StripGrids<TS, TC, MODE>& g = StripGrids<TS, TC, MODE>::get();
+      const int nf = L.nstrips_fast, ns = L.nstrips - L.nstrips_fast;
+      if (nf > 0)
+        k_strip<TS, TC, MODE, false><<<blocks_for((nf + 3) / 4, g.fast), 256, 0, s>>>(L.insts, L.strips, nf, par,
+                                                                                      step);
+      if (ns > 0)
+        k_strip<TS, TC, MODE, true><<<blocks_for((ns + 3) / 4, g.solid), 256, 0, s>>>(L.insts, L.strips + nf, ns,
+                                                                                      par, step);
+    }
+    tiles = L.tiles_rest > 0;
+  }
+  if (tiles)
+    FluxKernel<TS, TC, MODE, MOM, Sh>::fn<<<blocks_for(L.ntiles, grid_tiles), Sh::NT, 0, s>>>(L.insts, L.tiles,
+                                                                                             L.ntiles, par, step);
+  launch_phys<TS, TC, MODE, MOM, Sh>(L, s, par);
+}
+
 template <typename TS, typename TC>
 static void prepare_shapes() {
   (void)StepGrids<TS, TC, Shape0>::get();
@@ -1923,6 +1978,9 @@ static void prepare_shapes() {
   (void)StepGrids<TS, TC, Shape2>::get();
   (void)StepGrids<TS, TC, Shape3>::get();
   (void)StepGrids<TS, TC, Shape4>::get();
+  (void)StripGrids<TS, TC, 0>::get();
+  (void)StripGrids<TS, TC, 2>::get();
+  if constexpr (sizeof(TS) == 8) (void)StripGrids<TS, TC, 1>::get();
 }
 
 void prepare_launch(int fp64) {
@@ -1942,21 +2000,11 @@ static void per_eq_kind(const StepLaunch& L, F&& launch) {
   }
 }
 
-// The physical-addressing net fluxes of the flagged tiles, after the flux
-// kernel of the same MODE.
-template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
-static void launch_phys(const StepLaunch& L, hipStream_t s, int par) {
-  if (L.nphys > 0)
-    k_phys<TS, TC, MODE, MOM, Sh><<<L.ntiles < 1024 ? L.ntiles : 1024, 256, 0, s>>>(L.insts, L.tiles, L.ntiles, par,
-                                                                                  L.step_ctr);
-}
-
 template <typename TS, typename TC, typename Sh>
 static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer* tm) {
   const int* step = L.step_ctr;
   const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
   const int g1 = G.flux0, g3 = G.flux3;
-  const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if (tm) tm->begin(0, s);
   per_eq_kind<TS>(L, [&](auto te, int k) {
@@ -1970,8 +2018,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  FluxKernel<TS, TC, 0, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
-  launch_phys<TS, TC, 0, false, Sh>(L, s, par);
+  launch_flux<TS, TC, 0, false, Sh>(L, s, par, g1);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   per_eq_kind<TS>(L, [&](auto te, int k) {
@@ -1985,8 +2032,7 @@ static void launch_rk3(const StepLaunch& L, hipStream_t s, int par, KernelTimer*
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(4, s);
-  FluxKernel<TS, TC, 1, kMom3<TS>, Sh>::fn<<<blocks_for(n, g3), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
-  launch_phys<TS, TC, 1, kMom3<TS>, Sh>(L, s, par);
+  launch_flux<TS, TC, 1, kMom3<TS>, Sh>(L, s, par, g3);
   if (tm) tm->end(4, s);
 }
 
@@ -1998,7 +2044,6 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
   const StepGrids<TS, TC, Sh>& G = StepGrids<TS, TC, Sh>::get();
   const int g1 = FM == 0 ? G.flux0 : G.fluxe;
   const int* g2 = METHOD == 0 ? G.fin0 : (METHOD == 2 ? G.fin2 : G.fin3);
-  const int n = L.ntiles;
   const int wb = (L.nwalls + 255) / 256;
   if constexpr (METHOD == 0) {
     if (tm) tm->begin(0, s);
@@ -2014,8 +2059,7 @@ static void launch_one_stage(const StepLaunch& L, hipStream_t s, int par, Kernel
     if (tm) tm->end(1, s);
   }
   if (tm) tm->begin(2, s);
-  FluxKernel<TS, TC, FM, false, Sh>::fn<<<blocks_for(n, g1), Sh::NT, 0, s>>>(L.insts, L.tiles, n, par, step);
-  launch_phys<TS, TC, FM, false, Sh>(L, s, par);
+  launch_flux<TS, TC, FM, false, Sh>(L, s, par, g1);
   if (tm) tm->end(2, s);
   if (tm) tm->begin(3, s);
   per_eq_kind<TS>(L, [&](auto te, int k) {

@@ -30,6 +30,10 @@ struct StepLaunch {
   int nphys = 0;                    // tiles flagged kTilePhysical in `tiles` (k_phys runs when > 0)
   int eq_kinds = 1;                 // equilibrium precisions among the instances: bit 0 FP64,
                                     // bit 1 FP32 (deviation form; FP32 storage only)
+  const TileRef* strips = nullptr;  // strip work items (instances with InstDev::strips set):
+  int nstrips = 0;                  // items without a solid cell in their region first
+  int nstrips_fast = 0;             // (the first nstrips_fast), then those with one
+  int tiles_rest = 0;               // flux tiles of the instances not on strips
 };
 
 // Flux tiles of a list flagged kTilePhysical.
@@ -53,7 +57,8 @@ inline bool same_launch(const StepLaunch& a, const StepLaunch& b) {
   return a.insts == b.insts && a.tiles == b.tiles && a.ntiles == b.ntiles && a.walls == b.walls &&
          a.nwalls == b.nwalls && a.ptiles == b.ptiles && a.nptiles == b.nptiles && a.rk_method == b.rk_method &&
          a.step_ctr == b.step_ctr && a.flux_shape == b.flux_shape && a.res_partial == b.res_partial && a.ninst == b.ninst &&
-         a.nphys == b.nphys && a.eq_kinds == b.eq_kinds;
+         a.nphys == b.nphys && a.eq_kinds == b.eq_kinds && a.strips == b.strips && a.nstrips == b.nstrips &&
+         a.nstrips_fast == b.nstrips_fast && a.tiles_rest == b.tiles_rest;
 }
 
 // Optional per-kernel timing with HIP events recorded on the launch stream

@@ -368,7 +368,7 @@ Solver::~Solver() {
   for (void* p : {state[0], state[1], f2, f3, (void*)mom[0], (void*)mom[1], (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
                   (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
                   (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev, (void*)resid_dev,
-                  (void*)fails_dev, (void*)res_hist})
+                  (void*)fails_dev, (void*)res_hist, (void*)strips_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -450,6 +450,7 @@ static void fill_inst(const Solver& s, InstDev& d) {
   d.res_last = 0;
   d.ptile0 = 0;
   d.nptiles = s.nptiles;
+  d.strips = 0;  // set per launch group (run_group)
 }
 
 // Flux tiles of one shape with the per-tile solid flag (the tile's region
@@ -489,6 +490,72 @@ static std::vector<TileRef> build_flux_tiles(const Solver& s, int shape) {
       tiles.push_back(TileRef{0, tx, ty, (solid ? kTileSolid : 0) | (phys ? kTilePhysical : 0)});
     }
   return tiles;
+}
+
+// Strip work items of an instance whose cells all take the logical stencil
+// addressing (none otherwise): strip sx covers columns 58 sx .. 58 sx + 57
+// and row block ry rows 32 ry .. 32 ry + 31; an item is solid when its
+// region (3-cell halo included, conservatively at the edges) holds a solid
+// cell, as build_flux_tiles flags tiles.
+static void build_strips(const Solver& s, std::vector<TileRef>& fast, std::vector<TileRef>& solid) {
+  fast.clear();
+  solid.clear();
+  if (s.any_phys) return;
+  const int H = tile_halo();
+  std::vector<int> pre((size_t)(s.nx + 1) * (s.ny + 1), 0);
+  auto P = [&](int x, int y) -> int& { return pre[(size_t)y * (s.nx + 1) + x]; };
+  for (int y = 0; y < s.ny; y++)
+    for (int x = 0; x < s.nx; x++)
+      P(x + 1, y + 1) = P(x, y + 1) + P(x + 1, y) - P(x, y) + (type_at(s, x, y) == 2);
+  auto count = [&](int x0, int y0, int x1, int y1) {
+    x0 = std::max(x0, 0); y0 = std::max(y0, 0);
+    x1 = std::min(x1, s.nx); y1 = std::min(y1, s.ny);
+    if (x0 >= x1 || y0 >= y1) return 0;
+    return P(x1, y1) - P(x0, y1) - P(x1, y0) + P(x0, y0);
+  };
+  const bool any_edge_solid = count(0, 0, s.nx, H + 1) || count(0, s.ny - H - 1, s.nx, s.ny) ||
+                              count(0, 0, H + 1, s.ny) || count(s.nx - H - 1, 0, s.nx, s.ny);
+  for (int ry = 0; ry < (s.ny + kStripH - 1) / kStripH; ry++)
+    for (int sx = 0; sx < (s.nx + kStripW - 1) / kStripW; sx++) {
+      const int x0 = sx * kStripW, y0 = ry * kStripH;
+      bool sol = count(x0 - H, y0 - H, x0 + kStripW + H, y0 + kStripH + H) > 0;
+      const bool edge = x0 - H < 0 || y0 - H < 0 || x0 + kStripW + H > s.nx || y0 + kStripH + H > s.ny;
+      if (edge && any_edge_solid) sol = true;
+      (sol ? solid : fast).push_back(TileRef{0, sx, ry, sol ? kTileSolid : 0});
+    }
+}
+
+static int g_strips = 1;  // lbm_set_strips mode
+bool uqlbm::strips_enabled() { return g_strips != 0; }
+
+// A strip item is one wavefront's sequential march over 12 directions and
+// up to 32 rows, so a launch group takes the strip kernels only with enough
+// items to fill the GPU (two waves per SIMD of 256 CUs); smaller groups (one
+// small instance, the 512 x 256 plate) keep the tile kernels, which spread
+// an instance over many more workgroups.
+static constexpr int kStripMinItems = 2048;
+bool uqlbm::strips_for(int nstrips) {
+  return nstrips > 0 && (g_strips == 2 || (g_strips == 1 && nstrips >= kStripMinItems));
+}
+
+int uqlbm::group_strips(const std::vector<Solver*>& members, int shape, std::vector<TileRef>& out, int* tiles_rest) {
+  out.clear();
+  int nfast = 0, rest = 0;
+  for (size_t k = 0; k < members.size(); k++)
+    for (TileRef t : members[k]->strips_fast) {
+      t.inst = (int)k;
+      out.push_back(t);
+      nfast++;
+    }
+  for (size_t k = 0; k < members.size(); k++)
+    for (TileRef t : members[k]->strips_solid) {
+      t.inst = (int)k;
+      out.push_back(t);
+    }
+  for (Solver* s : members)
+    if (s->nstrips == 0) rest += (int)solver_tiles(*s, shape).size();
+  if (tiles_rest) *tiles_rest = rest;
+  return nfast;
 }
 
 const std::vector<TileRef>& uqlbm::solver_tiles(Solver& s, int shape) {
@@ -647,6 +714,15 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   }
   s->walls_host = std::move(walls);
   s->ptiles_host = std::move(ptiles);
+  build_strips(*s, s->strips_fast, s->strips_solid);
+  s->nstrips_fast = (int)s->strips_fast.size();
+  s->nstrips = s->nstrips_fast + (int)s->strips_solid.size();
+  if (s->nstrips) {
+    std::vector<TileRef> all(s->strips_fast);
+    all.insert(all.end(), s->strips_solid.begin(), s->strips_solid.end());
+    HIP_TRY(hipMalloc((void**)&s->strips_dev, all.size() * sizeof(TileRef)));
+    HIP_TRY(hipMemcpy(s->strips_dev, all.data(), all.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  }
   const int st = upload_inst(*s);
   if (st != LBM_OK) return st;
   *out = reinterpret_cast<lbm_solver*>(s.release());
@@ -785,6 +861,7 @@ int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, co
     table[k].res_last = members[k]->step + n;
     table[k].ptile0 = ptile0;
     table[k].nptiles = members[k]->nptiles;
+    table[k].strips = L.nstrips > 0 && members[k]->nstrips > 0 ? 1 : 0;
     ptile0 += members[k]->nptiles;
   }
   HIP_TRY(hipMemcpyAsync(table_dev, table.data(), table.size() * sizeof(InstDev), hipMemcpyHostToDevice, stream));
@@ -838,6 +915,18 @@ int uqlbm::run_group(const std::vector<Solver*>& members, InstDev* table_dev, co
 
 // Residual of the solver's last step on the device (cpuLBM.cpp:2322-2331):
 // `rows` holds ny + 1 doubles; the result is copied to *out.
+// The solver's strip list in its launch description (when strips are on).
+static void solver_strips(const Solver& s, StepLaunch& L) {
+  if (strips_for(s.nstrips)) {
+    L.strips = s.strips_dev;
+    L.nstrips = s.nstrips;
+    L.nstrips_fast = s.nstrips_fast;
+    L.tiles_rest = 0;
+  } else {
+    L.tiles_rest = s.ntiles;
+  }
+}
+
 static int device_residual(Solver& s, double* rows, double* out) {
   if (s.precision == LBM_PREC_FP64) launch_residual<double>(s.inst_dev, 0, 0, rows, s.nx, s.ny, s.stream);
   else launch_residual<float>(s.inst_dev, 0, 0, rows, s.nx, s.ny, s.stream);
@@ -857,6 +946,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   L.nphys = s.nphys;
+  solver_strips(s, L);
   L.eq_kinds = s.precision == LBM_PREC_FP64 ? 1 : (s.eq32 ? 2 : 1);
   std::vector<Solver*> one{&s};
   const int st = run_group(one, s.inst_dev, L, s.precision, s.stream, nsteps, s.graphs.get(), s.fails_dev, true);
@@ -925,6 +1015,7 @@ extern "C" int lbm_step_noalloc(lbm_solver* h, int32_t nsteps, int32_t want_resi
   StepLaunch L{s.inst_dev,    s.tiles_dev, s.ntiles,      s.walls_dev, s.nwalls,
                s.ptiles_dev, s.nptiles,   s.p.rk_method, s.step_ctr,  s.flux_shape};
   L.nphys = s.nphys;
+  solver_strips(s, L);
   L.eq_kinds = s.precision == LBM_PREC_FP64 ? 1 : (s.eq32 ? 2 : 1);
   prepare_launch(s.precision == LBM_PREC_FP64);
   std::vector<Solver*> one{&s};
@@ -1170,6 +1261,12 @@ extern "C" int lbm_set_kernel_timing(int32_t enable) {
   return LBM_OK;
 }
 
+extern "C" int lbm_set_strips(int32_t mode) {
+  if (mode < 0 || mode > 2) return fail_with(LBM_ERR_ARG, "strip mode must be 0, 1 or 2");
+  g_strips = mode;
+  return LBM_OK;
+}
+
 extern "C" int lbm_set_graphs(int32_t enable) {
   g_graphs_enabled = enable != 0;
   return LBM_OK;
@@ -1221,6 +1318,9 @@ int uqlbm::batch_set_shape(Batch& b, int shape) {
   HIP_TRY(hipMemcpy(b.tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   b.ntiles = (int)tiles.size();
   b.nphys = count_physical(tiles);
+  b.tiles_rest = 0;
+  for (Solver* m : b.members)
+    if (m->nstrips == 0) b.tiles_rest += (int)solver_tiles(*m, shape).size();
   b.eq_kinds = eq_kinds_of(b.precision == LBM_PREC_FP64, b.members.begin(), b.members.end(),
                            [](const Solver* m) { return m->eq32; });
   b.flux_shape = shape;
@@ -1254,6 +1354,13 @@ int uqlbm::batch_finish(Batch& b, lbm_batch** out) {
   }
   HIP_TRY(hipMalloc((void**)&b.ptiles_dev, ptiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(b.ptiles_dev, ptiles.data(), ptiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  std::vector<TileRef> strips;
+  b.nstrips_fast = group_strips(b.members, b.flux_shape, strips, nullptr);
+  b.nstrips = (int)strips.size();
+  if (!strips.empty()) {
+    HIP_TRY(hipMalloc((void**)&b.strips_dev, strips.size() * sizeof(TileRef)));
+    HIP_TRY(hipMemcpy(b.strips_dev, strips.data(), strips.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  }
   *out = reinterpret_cast<lbm_batch*>(new Batch(std::move(b)));
   return LBM_OK;
 }
@@ -1263,7 +1370,7 @@ uqlbm::Batch::~Batch() {
   for (Solver* s : members)
     if (--s->batch_refs == 0 && s->destroy_pending) delete s;
   for (void* p : {(void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr,
-                  (void*)fails_dev})
+                  (void*)fails_dev, (void*)strips_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -1273,6 +1380,14 @@ int uqlbm::batch_step(Batch& b, int nsteps) {
                b.ptiles_dev, b.nptiles,   b.rk_method, b.step_ctr,  b.flux_shape};
   L.nphys = b.nphys;
   L.eq_kinds = b.eq_kinds;
+  if (strips_for(b.nstrips)) {
+    L.strips = b.strips_dev;
+    L.nstrips = b.nstrips;
+    L.nstrips_fast = b.nstrips_fast;
+    L.tiles_rest = b.tiles_rest;
+  } else {
+    L.tiles_rest = b.ntiles;
+  }
   int st = run_group(b.members, b.inst_dev, L, b.precision, b.stream, nsteps, b.graphs.get(), b.fails_dev, true);
   for (Solver* s : b.members) {
     int st2 = upload_inst(*s);

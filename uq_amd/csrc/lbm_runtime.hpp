@@ -54,6 +54,7 @@ struct Solver {
   std::vector<TileRef> tiles_shape[kFluxShapes];  // flux tiles per shape, built on first use
   std::vector<WallRef> walls_host;
   std::vector<TileRef> ptiles_host;
+  std::vector<TileRef> strips_fast, strips_solid;  // strip items (none when a cell needs physical addressing)
 
   hipStream_t stream = nullptr;
   void* state[2] = {nullptr, nullptr};
@@ -74,6 +75,7 @@ struct Solver {
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
+  TileRef* strips_dev = nullptr;    // strips_fast then strips_solid
   double* macro_dev = nullptr;
   int* step_ctr = nullptr;          // device step counter of this solver's launches
   int flux_shape = 0;               // flux tile shape of tiles_dev
@@ -88,6 +90,7 @@ struct Solver {
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
   int ntiles = 0, nwalls = 0, nptiles = 0;
   int nphys = 0;  // flux tiles flagged kTilePhysical
+  int nstrips = 0, nstrips_fast = 0;
   size_t device_bytes = 0;
 
   int cur = 0;    // state buffer holding the current state
@@ -111,6 +114,7 @@ struct Batch {
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
+  TileRef* strips_dev = nullptr;
   int* step_ctr = nullptr;
   int* fails_dev = nullptr;         // members' failure steps gathered for one host copy
   int flux_shape = 0;
@@ -118,6 +122,7 @@ struct Batch {
   int ntiles = 0, nwalls = 0, nptiles = 0;
   int nphys = 0;  // flux tiles flagged kTilePhysical
   int eq_kinds = 1;  // equilibrium precisions among the members (StepLaunch::eq_kinds)
+  int nstrips = 0, nstrips_fast = 0, tiles_rest = 0;
   Batch() = default;
   Batch(Batch&& o) noexcept { *this = std::move(o); }
   Batch& operator=(Batch&& o) noexcept {
@@ -129,6 +134,7 @@ struct Batch {
     tiles_dev = o.tiles_dev;
     walls_dev = o.walls_dev;
     ptiles_dev = o.ptiles_dev;
+    strips_dev = o.strips_dev;
     step_ctr = o.step_ctr;
     fails_dev = o.fails_dev;
     flux_shape = o.flux_shape;
@@ -139,11 +145,15 @@ struct Batch {
     nptiles = o.nptiles;
     nphys = o.nphys;
     eq_kinds = o.eq_kinds;
+    nstrips = o.nstrips;
+    nstrips_fast = o.nstrips_fast;
+    tiles_rest = o.tiles_rest;
     o.stream = nullptr;
     o.inst_dev = nullptr;
     o.tiles_dev = nullptr;
     o.walls_dev = nullptr;
     o.ptiles_dev = nullptr;
+    o.strips_dev = nullptr;
     o.step_ctr = nullptr;
     o.fails_dev = nullptr;
     return *this;
@@ -152,6 +162,14 @@ struct Batch {
 };
 
 int fail_with(int status, const char* fmt, ...);
+// Strip flux kernels in use (lbm_set_strips; default on), and whether a
+// launch group with `nstrips` strip items takes them.
+bool strips_enabled();
+bool strips_for(int nstrips);
+// Strip list of a launch group (member k's items with instance index k,
+// items without solids first; returns how many those are) and the number
+// of flux tiles of shape `shape` of the members not on strips.
+int group_strips(const std::vector<Solver*>& members, int shape, std::vector<TileRef>& out, int* tiles_rest);
 const std::vector<TileRef>& solver_tiles(Solver& s, int shape);
 int default_flux_shape(int precision);
 int batch_set_shape(Batch& b, int shape);

@@ -65,8 +65,10 @@ struct Server {
   TileRef* tiles_dev = nullptr;
   WallRef* walls_dev = nullptr;
   TileRef* ptiles_dev = nullptr;
-  size_t tiles_cap = 0, walls_cap = 0, ptiles_cap = 0;
+  TileRef* strips_dev = nullptr;
+  size_t tiles_cap = 0, walls_cap = 0, ptiles_cap = 0, strips_cap = 0;
   int ntiles = 0, nwalls = 0, nptiles = 0;
+  int nstrips = 0, nstrips_fast = 0, tiles_rest = 0;  // strip list of the active requests (StepLaunch)
   int nphys = 0;  // flux tiles flagged kTilePhysical
   int eq_kinds = 1;  // equilibrium precisions among the active requests (StepLaunch::eq_kinds)
   bool lists_dirty = true;
@@ -89,7 +91,7 @@ Server::~Server() {
   graphs.clear();
   for (auto& a : active) delete a.solver;
   for (void* p : {(void*)table_dev, (void*)tiles_dev, (void*)walls_dev, (void*)ptiles_dev, (void*)step_ctr,
-                  (void*)fails_dev, (void*)res_partial})
+                  (void*)fails_dev, (void*)res_partial, (void*)strips_dev})
     if (p) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -129,6 +131,20 @@ static int rebuild_lists(Server& sv) {
     sv.walls_cap = std::max(walls.size(), sv.walls_cap * 2);
     SV_TRY(hipMalloc((void**)&sv.walls_dev, sv.walls_cap * sizeof(WallRef)));
   }
+  std::vector<Solver*> members;
+  members.reserve(sv.active.size());
+  for (auto& a : sv.active) members.push_back(a.solver);
+  std::vector<TileRef> strips;
+  sv.nstrips_fast = group_strips(members, sv.flux_shape, strips, &sv.tiles_rest);
+  sv.nstrips = (int)strips.size();
+  if (strips.size() > sv.strips_cap) {
+    if (sv.strips_dev) SV_TRY(hipFree(sv.strips_dev));
+    sv.strips_cap = std::max(strips.size(), sv.strips_cap * 2);
+    SV_TRY(hipMalloc((void**)&sv.strips_dev, sv.strips_cap * sizeof(TileRef)));
+  }
+  if (!strips.empty())
+    SV_TRY(hipMemcpyAsync(sv.strips_dev, strips.data(), strips.size() * sizeof(TileRef), hipMemcpyHostToDevice,
+                          sv.stream));
   if (ptiles.size() > sv.ptiles_cap) {
     if (sv.ptiles_dev) SV_TRY(hipFree(sv.ptiles_dev));
     if (sv.res_partial) SV_TRY(hipFree(sv.res_partial));
@@ -349,6 +365,14 @@ extern "C" int lbm_server_tick(lbm_server* h, int32_t tick_steps, int32_t* activ
                  sv.ptiles_dev, sv.nptiles,   sv.rk_method, sv.step_ctr,  sv.flux_shape};
     L.nphys = sv.nphys;
     L.eq_kinds = sv.eq_kinds;
+    if (strips_for(sv.nstrips)) {
+      L.strips = sv.strips_dev;
+      L.nstrips = sv.nstrips;
+      L.nstrips_fast = sv.nstrips_fast;
+      L.tiles_rest = sv.tiles_rest;
+    } else {
+      L.tiles_rest = sv.ntiles;
+    }
     if (track) {
       L.res_partial = sv.res_partial;
       L.ninst = (int)members.size();

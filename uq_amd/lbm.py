@@ -59,6 +59,7 @@ def lib():
     L.lbm_batch_destroy.restype = None
     L.lbm_set_kernel_timing.argtypes = [i32]
     L.lbm_set_graphs.argtypes = [i32]
+    L.lbm_set_strips.argtypes = [i32]
     L.lbm_get_totals.argtypes = [vp, P(d)]
     L.lbm_set_flux_shape.argtypes = [i32, i32]
     L.lbm_set_viscosity.argtypes = [vp, i32, d, d, d]
@@ -115,6 +116,14 @@ def set_graphs(enable: bool) -> None:
     """Replay captured HIP graphs of the step sequence (default) or launch
     every kernel directly."""
     lib().lbm_set_graphs(1 if enable else 0)
+
+
+def set_strips(mode) -> None:
+    """Flux kernels of instances with logical addressing: 0 (or False) tiles
+    only, 1 (or True; default) strips for launch groups large enough to fill
+    the GPU, 2 strips wherever possible."""
+    mode = int(mode)
+    _check(lib().lbm_set_strips(mode), "lbm_set_strips")
 
 
 def last_timing() -> dict:
