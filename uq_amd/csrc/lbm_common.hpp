@@ -99,8 +99,8 @@ constexpr int kResHist = 20;  // residual history slots per tracked instance
 
 // One tile of one instance: the unit of work of a block.
 // Strip work items of the strip flux kernels (TileRef with tx = strip index,
-// ty = row block): kStripW own columns (with a 3-column halo on each side,
-// 64 lanes) by up to kStripH rows.
+// ty = first row, flags = kTileSolid | row count << 8): kStripW own columns
+// (with a 3-column halo on each side, 64 lanes) by up to kStripH rows.
 constexpr int kStripW = 58, kStripH = 32;
 
 struct TileRef {

@@ -1928,8 +1928,12 @@ static void launch_phys(const StepLaunch& L, hipStream_t s, int par) {
                                                                                   L.step_ctr);
 }
 
-// Persistent grids of the strip kernels of one storage/compute type and
-// MODE (without and with solids).
+// Compute type of the strip kernels: FP64 for FP64 storage; for FP32
+// storage the precision of the instance's equilibria (FP32 for instances
+// near their reference equilibrium, FP64 for the others: on the 4096^2
+// Riemann deck FP32 stencil arithmetic on strips ended the mixed run at
+// step 1939, FP64 arithmetic completes the 2000 steps), each kind of
+// instance taken by its own launch over the same item list.
 template <typename TS, typename TC, int MODE>
 struct StripGrids {
   int fast = 0, solid = 0;
@@ -1944,6 +1948,18 @@ struct StripGrids {
   }
 };
 
+template <typename TS, typename TC, int MODE>
+static void launch_strips(const StepLaunch& L, hipStream_t s, int par) {
+  const StripGrids<TS, TC, MODE>& g = StripGrids<TS, TC, MODE>::get();
+  const int nf = L.nstrips_fast, ns = L.nstrips - L.nstrips_fast;
+  if (nf > 0)
+    k_strip<TS, TC, MODE, false><<<blocks_for((nf + 3) / 4, g.fast), 256, 0, s>>>(L.insts, L.strips, nf, par,
+                                                                                  L.step_ctr);
+  if (ns > 0)
+    k_strip<TS, TC, MODE, true><<<blocks_for((ns + 3) / 4, g.solid), 256, 0, s>>>(L.insts, L.strips + nf, ns, par,
+                                                                                  L.step_ctr);
+}
+
 // One flux stage of MODE: the strip kernels for the instances on strips and
 // the tile kernel for the others (every instance for the moment-summing
 // stage-3 flux of FP32 storage, which has no strip form), then the
@@ -1954,14 +1970,12 @@ static void launch_flux(const StepLaunch& L, hipStream_t s, int par, int grid_ti
   bool tiles = true;
   if constexpr (!MOM) {
     if (L.nstrips > 0) {
-      const StripGrids<TS, TC, MODE>& g = StripGrids<TS, TC, MODE>::get();
-      const int nf = L.nstrips_fast, ns = L.nstrips - L.nstrips_fast;
-      if (nf > 0)
-        k_strip<TS, TC, MODE, false><<<blocks_for((nf + 3) / 4, g.fast), 256, 0, s>>>(L.insts, L.strips, nf, par,
-                                                                                      step);
-      if (ns > 0)
-        k_strip<TS, TC, MODE, true><<<blocks_for((ns + 3) / 4, g.solid), 256, 0, s>>>(L.insts, L.strips + nf, ns,
-                                                                                      par, step);
+      if constexpr (sizeof(TS) == 4) {
+        if (L.eq_kinds & 2) launch_strips<TS, float, MODE>(L, s, par);
+        if (L.eq_kinds & 1) launch_strips<TS, double, MODE>(L, s, par);
+      } else {
+        launch_strips<TS, double, MODE>(L, s, par);
+      }
     }
     tiles = L.tiles_rest > 0;
   }
@@ -1978,9 +1992,14 @@ static void prepare_shapes() {
   (void)StepGrids<TS, TC, Shape2>::get();
   (void)StepGrids<TS, TC, Shape3>::get();
   (void)StepGrids<TS, TC, Shape4>::get();
-  (void)StripGrids<TS, TC, 0>::get();
-  (void)StripGrids<TS, TC, 2>::get();
-  if constexpr (sizeof(TS) == 8) (void)StripGrids<TS, TC, 1>::get();
+  (void)StripGrids<TS, double, 0>::get();
+  (void)StripGrids<TS, double, 2>::get();
+  if constexpr (sizeof(TS) == 8) {
+    (void)StripGrids<TS, double, 1>::get();
+  } else {
+    (void)StripGrids<TS, float, 0>::get();
+    (void)StripGrids<TS, float, 2>::get();
+  }
 }
 
 void prepare_launch(int fp64) {

@@ -493,36 +493,57 @@ static std::vector<TileRef> build_flux_tiles(const Solver& s, int shape) {
 }
 
 // Strip work items of an instance whose cells all take the logical stencil
-// addressing (none otherwise): strip sx covers columns 58 sx .. 58 sx + 57
-// and row block ry rows 32 ry .. 32 ry + 31; an item is solid when its
-// region (3-cell halo included, conservatively at the edges) holds a solid
-// cell, as build_flux_tiles flags tiles.
+// addressing (none otherwise). Strip sx covers columns 58 sx .. 58 sx + 57;
+// its rows are grouped into items of at most 32 rows. A row is solid when
+// the region its cells' stencils read (the strip's 64 columns x0 - 3 ..
+// x0 + 60 by rows y - 3 .. y + 3, each position mapped as the kernels map
+// it: periodic, mirrored or clamped) holds a solid cell; consecutive rows
+// of the same kind form an item, so that only the rows near a solid take
+// the solid path. TileRef: tx = strip index, ty = first row,
+// flags = kTileSolid (solid rows) | row count << 8.
 static void build_strips(const Solver& s, std::vector<TileRef>& fast, std::vector<TileRef>& solid) {
   fast.clear();
   solid.clear();
   if (s.any_phys) return;
   const int H = tile_halo();
-  std::vector<int> pre((size_t)(s.nx + 1) * (s.ny + 1), 0);
-  auto P = [&](int x, int y) -> int& { return pre[(size_t)y * (s.nx + 1) + x]; };
-  for (int y = 0; y < s.ny; y++)
-    for (int x = 0; x < s.nx; x++)
-      P(x + 1, y + 1) = P(x, y + 1) + P(x + 1, y) - P(x, y) + (type_at(s, x, y) == 2);
-  auto count = [&](int x0, int y0, int x1, int y1) {
-    x0 = std::max(x0, 0); y0 = std::max(y0, 0);
-    x1 = std::min(x1, s.nx); y1 = std::min(y1, s.ny);
-    if (x0 >= x1 || y0 >= y1) return 0;
-    return P(x1, y1) - P(x0, y1) - P(x1, y0) + P(x0, y0);
-  };
-  const bool any_edge_solid = count(0, 0, s.nx, H + 1) || count(0, s.ny - H - 1, s.nx, s.ny) ||
-                              count(0, 0, H + 1, s.ny) || count(s.nx - H - 1, 0, s.nx, s.ny);
-  for (int ry = 0; ry < (s.ny + kStripH - 1) / kStripH; ry++)
-    for (int sx = 0; sx < (s.nx + kStripW - 1) / kStripW; sx++) {
-      const int x0 = sx * kStripW, y0 = ry * kStripH;
-      bool sol = count(x0 - H, y0 - H, x0 + kStripW + H, y0 + kStripH + H) > 0;
-      const bool edge = x0 - H < 0 || y0 - H < 0 || x0 + kStripW + H > s.nx || y0 + kStripH + H > s.ny;
-      if (edge && any_edge_solid) sol = true;
-      (sol ? solid : fast).push_back(TileRef{0, sx, ry, sol ? kTileSolid : 0});
+  bool any_solid = false;
+  for (size_t k = 0; k < s.ctype_host.size() && !any_solid; k++) any_solid = s.ctype_host[k] == 2;
+  const int nsx = (s.nx + kStripW - 1) / kStripW;
+  int m;
+  bool cl;
+  std::vector<int> xs(64), ys(s.ny + 2 * H);
+  for (int j = 0; j < s.ny + 2 * H; j++) ys[j] = ref_map(s, j - H, 1, m, cl);
+  std::vector<char> rs(s.ny + 2 * H, 0), rowsolid(s.ny, 0);
+  for (int sx = 0; sx < nsx; sx++) {
+    const int x0 = sx * kStripW;
+    if (any_solid) {
+      for (int l = 0; l < 64; l++) xs[l] = ref_map(s, x0 - H + l, 0, m, cl);
+      for (int j = 0; j < s.ny + 2 * H; j++) {  // mapped row j - H across the strip's columns
+        bool any = false;
+        for (int l = 0; l < 64 && !any; l++) any = type_at(s, xs[l], ys[j]) == 2;
+        rs[j] = any;
+      }
+      for (int y = 0; y < s.ny; y++) {
+        bool any = false;
+        for (int k = 0; k <= 2 * H && !any; k++) any = rs[y + k];  // rows y - H .. y + H
+        rowsolid[y] = any;
+      }
     }
+    for (int y = 0; y < s.ny;) {
+      const bool sol = rowsolid[y];
+      int n = 1;
+      while (y + n < s.ny && n < kStripH && (bool)rowsolid[y + n] == sol) n++;
+      (sol ? solid : fast).push_back(TileRef{0, sx, y, (sol ? kTileSolid : 0) | (n << 8)});
+      y += n;
+    }
+  }
+  // row-major order (first row, then strip): the waves of a workgroup take
+  // consecutive items, side by side strips of the same rows, whose halo
+  // columns they then share in L2 (4096^2 Riemann stage-2 and stage-3 flux
+  // 1.75 and 2.17 ms in this order, 2.18 and 2.74 ms strip by strip)
+  auto row_major = [](const TileRef& a, const TileRef& b) { return a.ty != b.ty ? a.ty < b.ty : a.tx < b.tx; };
+  std::stable_sort(fast.begin(), fast.end(), row_major);
+  std::stable_sort(solid.begin(), solid.end(), row_major);
 }
 
 static int g_strips = 1;  // lbm_set_strips mode

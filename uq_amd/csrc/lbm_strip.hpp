@@ -104,10 +104,10 @@ __device__ __forceinline__ void strip_line(TC v, const TC (&Vf)[5], const TC (&V
 }
 
 // Wall closure face of a solid owner (face_flux): alpha |e| f, for FP32
-// storage |e| (alpha dev + (alpha - 1) ref).
-template <typename TC>
+// storage (deviations dev = f - ref staged) |e| (alpha dev + (alpha - 1) ref).
+template <typename TS, typename TC>
 __device__ __forceinline__ void strip_wall(TC a, TC v, TC wf, TC wh, TC rf, TC rh, TC& vf, TC& vh) {
-  if constexpr (sizeof(TC) == 4) {
+  if constexpr (sizeof(TS) == 4) {
     vf = v * (a * wf + (a - TC(1)) * rf);
     vh = v * (a * wh + (a - TC(1)) * rh);
   } else {
@@ -231,7 +231,7 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
       // the owner's wall closure factor, staged by the owner lane
       const TC ao = SX > 0 ? lane_shift<kDppShr1>(r.a) : r.a;
       if (sld[2]) {  // solid owner: wall closure
-        strip_wall(ao, vx, Lf[2], Lh[2], rf, rh, vf, vh);
+        strip_wall<TS>(ao, vx, Lf[2], Lh[2], rf, rh, vf, vh);
         return;
       }
     }
@@ -252,7 +252,7 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
     }
     if constexpr (SOLID) {
       if (sld[2]) {
-        strip_wall(W[SY > 0 ? 2 : 3].a, vy, Lf[2], Lh[2], rf, rh, vf, vh);
+        strip_wall<TS>(W[SY > 0 ? 2 : 3].a, vy, Lf[2], Lh[2], rf, rh, vf, vh);
         return;
       }
     }
@@ -405,8 +405,8 @@ __device__ __forceinline__ void strip_dirs(const InstDev& in, const StripInst<TS
     const int sy = c_lat_y[I] > 0 ? 1 : (c_lat_y[I] < 0 ? -1 : 0);
     // reference equilibrium of direction I for the wall faces of FP32
     // storage (a mirror image of I has the same reference: it depends on |e|)
-    const TC rf = sizeof(TS) == 4 ? TC(float(in.ref[I])) : TC(0);
-    const TC rh = sizeof(TS) == 4 ? TC(float(in.ref[kQ + I])) : TC(0);
+    const TC rf = sizeof(TS) == 4 ? TC(in.ref[I]) : TC(0);
+    const TC rh = sizeof(TS) == 4 ? TC(in.ref[kQ + I]) : TC(0);
     if (sy == 0) {
       if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, 0>(si, C, rf, rh, I, y0, nrows);
       else strip_dir<TS, TC, MODE, SOLID, W5, -1, 0>(si, C, rf, rh, I, y0, nrows);
@@ -427,8 +427,7 @@ __device__ __forceinline__ void strip_dirs(const InstDev& in, const StripInst<TS
 template <typename TS, typename TC, int MODE, bool SOLID>
 __device__ __forceinline__ void strip_item(const InstDev& in, const TileRef& sr, int par) {
   const int lane = threadIdx.x & 63;
-  const int x0 = sr.tx * kStripW, y0 = sr.ty * kStripH;
-  const int nrows = in.ny - y0 < kStripH ? in.ny - y0 : kStripH;
+  const int x0 = sr.tx * kStripW, y0 = sr.ty, nrows = sr.flags >> 8;
   StripInst<TS> si;
   si.src = (gptr<const TS>)(MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]));
   si.dst = (gptr<TS>)(MODE == 1 ? in.state[1 ^ in.parity0 ^ par] : in.dflux);
@@ -461,10 +460,12 @@ __device__ __forceinline__ void strip_item(const InstDev& in, const TileRef& sr,
 }
 
 // Persistent grid of 4-wave blocks; wave w of the (XCD-renumbered) grid
-// takes items w, w + 4 G, ... of the list. At most 128 registers (4 waves
-// per SIMD) without solids, 168 (3 waves) with them.
+// takes items w, w + 4 G, ... of the list. At most 168 registers (3 waves
+// per SIMD): capped at 128 for a fourth wave the FP64 kernels spill (up to
+// 73 slots) and run slower (4096^2 Riemann stage-2 and stage-3 flux 2.19
+// and 2.82 ms at 4 waves, 1.77 and 2.18 ms at 3).
 template <typename TS, typename TC, int MODE, bool SOLID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SOLID ? 3 : 4))) void k_strip(const InstDev* insts, const TileRef* strips, int nstrips, int par,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_strip(const InstDev* insts, const TileRef* strips, int nstrips, int par,
                                                const int* stepp) {
   const int step = *stepp;
   const int wave = __builtin_amdgcn_readfirstlane(first_tile() * 4 + (int)(threadIdx.x >> 6));
@@ -473,6 +474,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SOLID ? 3 :
     const TileRef sr = strips[it];
     const InstDev& in = insts[sr.inst];
     if (frozen_by_failure(in, step)) continue;
+    if (!eq_kernel_for<TS, TC>(in)) continue;  // FP32 storage: the other compute type's instance
     strip_item<TS, TC, MODE, SOLID>(in, sr, par);
   }
 }
