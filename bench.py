@@ -69,19 +69,20 @@ def kernel_bytes_per_lup(es: int) -> dict:
 
 
 def measured_traffic(kernel: str, workload: str, shape: int):
-    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes committed under profiles/
-    (copy-calibrated): the pointwise kernels' records hold for every flux
-    tile shape, the flux kernels' are keyed `kernel@shape`. None when no
-    pass covers the kernel at that shape."""
+    """HBM bytes per launch (per step, for a stage of several launches) of
+    `kernel` on `workload`, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes committed under profiles/ (copy-calibrated): records of the
+    tile flux kernels are keyed `kernel@shape`, those of the pointwise
+    kernels and of the strip flux stages (which do not depend on the tile
+    shape) by the kernel alone. None when no pass covers it."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
             tab = json.load(fh)
     except (OSError, ValueError):
         return None
-    key = f"{kernel}@{shape}" if kernel.startswith("flux") else kernel
-    rec = tab.get(workload, {}).get(key)
+    recs = tab.get(workload, {})
+    rec = recs.get(f"{kernel}@{shape}") or recs.get(kernel)
     return None if rec is None else rec.get("bytes_per_launch")
 
 

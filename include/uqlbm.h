@@ -282,10 +282,11 @@ int lbm_set_graphs(int32_t enable);
  * stage on the tile kernels; 1 (default): strips for launch groups with at
  * least 2048 strip items (enough to fill the GPU; e.g. a 4096^2 instance or
  * the 256-member ensemble), tiles for smaller groups; 2: strips wherever an
- * instance can take them. Results agree to FP64 (FP32) rounding, not
- * bitwise: the two order a few operations differently. Applies to step
- * calls made afterwards; LBM_ERR_ARG for another mode. No counterpart in
- * the reference.
+ * instance can take them. Results are bitwise equal either way (FP64, and
+ * LBM_PREC_MIXED instances near their reference equilibrium; mixed
+ * instances far from it compute their strip stencils in FP64, the tile
+ * kernels in FP32). Applies to step calls made afterwards; LBM_ERR_ARG for
+ * another mode. No counterpart in the reference.
  */
 int lbm_set_strips(int32_t mode);
 

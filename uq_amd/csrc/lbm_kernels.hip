@@ -921,6 +921,7 @@ __device__ __forceinline__ void store_slots(StageBuf<Sh, TS>& B, const SlotVals<
 // stays in registers.)
 template <int L, typename TC>
 __device__ __forceinline__ void extrapolate(const int* sld, const TC* V, TC* S) {
+#pragma clang fp contract(off)
   bool found = false;
 #pragma unroll
   for (int j = 0; j < L - 1; j++) {
@@ -949,6 +950,7 @@ template <int AX, typename Sh, typename TC>
 __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, const double* alpha, int method,
                                           bool solids, int lx, int ly, int sg, TC v, TC rf, TC rh, TC& vf,
                                           TC& vh) {
+#pragma clang fp contract(off)  // the wall faces round as the strip kernels' do
   const int dx = AX == 0 ? sg : 0, dy = AX == 0 ? 0 : sg;
   if (!solids) {
     if (method == 1) {
@@ -1277,6 +1279,9 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   }
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
+    // no contraction: the net flux and the state update round as the strip
+    // kernels (and the oracle) round them, so both paths agree bit for bit
+#pragma clang fp contract(off)
     if (!W.fluid[q]) continue;
     const int ci = t + q * Sh::NT, tx = ci % Sh::TX, ty = ci / Sh::TX;
     // reference cFLUX (cpuLBM.cpp:1436-1470): (dt/dx)(Fx_out - Fx_in) +

@@ -107,6 +107,7 @@ __device__ __forceinline__ void strip_line(TC v, const TC (&Vf)[5], const TC (&V
 // storage (deviations dev = f - ref staged) |e| (alpha dev + (alpha - 1) ref).
 template <typename TS, typename TC>
 __device__ __forceinline__ void strip_wall(TC a, TC v, TC wf, TC wh, TC rf, TC rh, TC& vf, TC& vh) {
+#pragma clang fp contract(off)  // rounds as the tile kernels' wall faces
   if constexpr (sizeof(TS) == 4) {
     vf = v * (a * wf + (a - TC(1)) * rf);
     vh = v * (a * wh + (a - TC(1)) * rh);
@@ -307,6 +308,9 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   }
   // row y from its prefetched data (registers only: no memory access)
   auto row = [&](int y, const Pre& cc) {
+    // no contraction: the net flux and the state update round as the tile
+    // kernels (and the oracle) round them, so both paths agree bit for bit
+#pragma clang fp contract(off)
     TC Gf = 0, Gh = 0, Ff = 0, Fh = 0, Nf = 0, Nh = 0;
     SPos<TC> xr = cc.p;
     if constexpr (HY) {
