@@ -291,6 +291,16 @@ int lbm_set_graphs(int32_t enable);
 int lbm_set_strips(int32_t mode);
 
 /*
+ * Device buffers of destroyed solvers are kept (up to 32 GiB per process)
+ * for later solvers of the same sizes, so that a served request stream
+ * creates and destroys solvers without hipMalloc and hipFree; a solver
+ * whose allocation finds the device full first returns the kept buffers of
+ * its device. lbm_release_cached returns every kept buffer now. No
+ * counterpart in the reference.
+ */
+void lbm_release_cached(void);
+
+/*
  * Continuous-batching server. Requests are queued host-side; a request is
  * admitted (device state allocated and initialised) when a slot is free and
  * its device footprint fits within hbm_budget_bytes. Each tick advances

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -363,14 +364,91 @@ static void set_reference(Solver& s) {
   s.ref_t = q.t;
 }
 
+// Device buffers of destroyed solvers, kept per device and size for the
+// next solver that needs the same (a served request stream creates and
+// destroys solvers of recurring sizes; hipMalloc and hipFree, which waits
+// for the whole device, leave the serving loop). At most kCacheMax bytes
+// are kept; a hipMalloc that finds the device full returns the device's
+// kept buffers first (lbm_release_cached returns them on request).
+namespace {
+constexpr size_t kCacheMax = size_t(32) << 30;
+struct BufferCache {
+  std::mutex mu;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free;
+  size_t bytes = 0;
+  void drop(int dev) {  // with mu held
+    for (auto it = free.begin(); it != free.end();) {
+      if (dev >= 0 && it->first.first != dev) {
+        ++it;
+        continue;
+      }
+      for (void* p : it->second) (void)hipFree(p);
+      bytes -= it->first.second * it->second.size();
+      it = free.erase(it);
+    }
+  }
+};
+BufferCache& buffer_cache() {
+  static BufferCache* c = new BufferCache();  // never destroyed: outlives every solver
+  return *c;
+}
+}  // namespace
+
+hipError_t uqlbm::solver_alloc(Solver& s, void** p, size_t bytes) {
+  *p = nullptr;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  BufferCache& c = buffer_cache();
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto it = c.free.find({dev, bytes});
+    if (it != c.free.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      c.bytes -= bytes;
+    }
+  }
+  if (!*p) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      {
+        std::lock_guard<std::mutex> lock(c.mu);
+        c.drop(dev);
+      }
+      e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) return e;
+  }
+  s.owned.push_back({*p, bytes});
+  return hipSuccess;
+}
+
 Solver::~Solver() {
   if (graphs) graphs->clear();
-  for (void* p : {state[0], state[1], f2, f3, (void*)mom[0], (void*)mom[1], (void*)ctype, (void*)alpha, (void*)fail, (void*)ids,
-                  (void*)props_dev, (void*)inst_dev, (void*)tiles_dev, (void*)walls_dev, (void*)macro_dev, dflux,
-                  (void*)ptiles_dev, (void*)step_ctr, (void*)totals_dev, (void*)spacing_dev, (void*)resid_dev,
-                  (void*)fails_dev, (void*)res_hist, (void*)strips_dev})
-    if (p) (void)hipFree(p);
+  // no work of this solver may still use a buffer that the next solver gets
+  if (stream) (void)hipStreamSynchronize(stream);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  BufferCache& c = buffer_cache();
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    for (auto& ob : owned) {
+      if (c.bytes + ob.second <= kCacheMax) {
+        c.free[{dev, ob.second}].push_back(ob.first);
+        c.bytes += ob.second;
+      } else {
+        (void)hipFree(ob.first);
+      }
+    }
+  }
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" void lbm_release_cached(void) {
+  BufferCache& c = buffer_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.drop(-1);
 }
 
 static void fill_inst(const Solver& s, InstDev& d) {
@@ -674,25 +752,25 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   const size_t plane = (size_t)s->pitch * ny;
   const size_t es = elem_size(precision);
   HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  for (void** b : {&s->state[0], &s->state[1], &s->f2, &s->f3}) HIP_TRY(hipMalloc(b, plane * kPlanes * es));
-  HIP_TRY(hipMalloc(&s->dflux, plane * dflux_planes() * es));
+  for (void** b : {&s->state[0], &s->state[1], &s->f2, &s->f3}) HIP_TRY(solver_alloc(*s, b, plane * kPlanes * es));
+  HIP_TRY(solver_alloc(*s, &s->dflux, plane * dflux_planes() * es));
   if (precision != LBM_PREC_FP64) {
     for (double** m : {&s->mom[0], &s->mom[1]}) {
-      HIP_TRY(hipMalloc((void**)m, plane * 4 * sizeof(double)));
+      HIP_TRY(solver_alloc(*s, (void**)m, plane * 4 * sizeof(double)));
       HIP_TRY(hipMemset(*m, 0, plane * 4 * sizeof(double)));
     }
     set_reference(*s);
     s->eq32 = near_reference(*s);
   }
-  HIP_TRY(hipMalloc((void**)&s->ctype, plane));
-  HIP_TRY(hipMalloc((void**)&s->alpha, plane * sizeof(double)));
-  HIP_TRY(hipMalloc((void**)&s->fail, sizeof(int)));
-  HIP_TRY(hipMalloc((void**)&s->ids, plane * sizeof(int)));
-  HIP_TRY(hipMalloc((void**)&s->props_dev, s->props.size() * 4 * sizeof(double)));
-  HIP_TRY(hipMalloc((void**)&s->inst_dev, sizeof(InstDev)));
-  HIP_TRY(hipMalloc((void**)&s->macro_dev, plane * 5 * sizeof(double)));
-  HIP_TRY(hipMalloc((void**)&s->step_ctr, sizeof(int)));
-  HIP_TRY(hipMalloc((void**)&s->fails_dev, sizeof(int)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->ctype, plane));
+  HIP_TRY(solver_alloc(*s, (void**)&s->alpha, plane * sizeof(double)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->fail, sizeof(int)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->ids, plane * sizeof(int)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->props_dev, s->props.size() * 4 * sizeof(double)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->inst_dev, sizeof(InstDev)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->macro_dev, plane * 5 * sizeof(double)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->step_ctr, sizeof(int)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->fails_dev, sizeof(int)));
   s->device_bytes = solver_footprint(nx, ny, precision);
 
   std::vector<uint8_t> ct(plane, 2);
@@ -725,12 +803,12 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   s->nphys = count_physical(tiles);
   s->nwalls = (int)walls.size();
   s->nptiles = (int)ptiles.size();
-  HIP_TRY(hipMalloc((void**)&s->ptiles_dev, ptiles.size() * sizeof(TileRef)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->ptiles_dev, ptiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(s->ptiles_dev, ptiles.data(), ptiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc((void**)&s->tiles_dev, tiles.size() * sizeof(TileRef)));
+  HIP_TRY(solver_alloc(*s, (void**)&s->tiles_dev, tiles.size() * sizeof(TileRef)));
   HIP_TRY(hipMemcpy(s->tiles_dev, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   if (!walls.empty()) {
-    HIP_TRY(hipMalloc((void**)&s->walls_dev, walls.size() * sizeof(WallRef)));
+    HIP_TRY(solver_alloc(*s, (void**)&s->walls_dev, walls.size() * sizeof(WallRef)));
     HIP_TRY(hipMemcpy(s->walls_dev, walls.data(), walls.size() * sizeof(WallRef), hipMemcpyHostToDevice));
   }
   s->walls_host = std::move(walls);
@@ -741,7 +819,7 @@ extern "C" int lbm_create(const lbm_params* params, int32_t nx, int32_t ny, cons
   if (s->nstrips) {
     std::vector<TileRef> all(s->strips_fast);
     all.insert(all.end(), s->strips_solid.begin(), s->strips_solid.end());
-    HIP_TRY(hipMalloc((void**)&s->strips_dev, all.size() * sizeof(TileRef)));
+    HIP_TRY(solver_alloc(*s, (void**)&s->strips_dev, all.size() * sizeof(TileRef)));
     HIP_TRY(hipMemcpy(s->strips_dev, all.data(), all.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   }
   const int st = upload_inst(*s);
@@ -977,7 +1055,7 @@ extern "C" int lbm_step(lbm_solver* h, int32_t nsteps, int32_t want_residual, do
   if (want_residual && residual_out) {
     // the previous state is still in the other state buffer: the RMS change
     // of T over the last step is reduced on the device
-    if (!s.resid_dev) HIP_TRY(hipMalloc((void**)&s.resid_dev, ((size_t)s.ny + 1) * sizeof(double)));
+    if (!s.resid_dev) HIP_TRY(solver_alloc(s, (void**)&s.resid_dev, ((size_t)s.ny + 1) * sizeof(double)));
     const int st3 = device_residual(s, s.resid_dev, residual_out);
     if (st3 != LBM_OK) return st3;
   }
@@ -1072,7 +1150,7 @@ extern "C" int lbm_get_totals(lbm_solver* h, double* out) {
   if (!h || !out) return fail_with(LBM_ERR_ARG, "null argument");
   Solver& s = *reinterpret_cast<Solver*>(h);
   if (!s.inited) return fail_with(LBM_ERR_STATE, "lbm_get_totals before lbm_init");
-  if (!s.totals_dev) HIP_TRY(hipMalloc((void**)&s.totals_dev, totals_scratch_doubles() * sizeof(double)));
+  if (!s.totals_dev) HIP_TRY(solver_alloc(s, (void**)&s.totals_dev, totals_scratch_doubles() * sizeof(double)));
   const long long cells = (long long)s.pitch * s.ny;
   if (s.precision == LBM_PREC_FP64) launch_totals<double>(s.inst_dev, 0, 0, s.totals_dev, cells, s.stream);
   else launch_totals<float>(s.inst_dev, 0, 0, s.totals_dev, cells, s.stream);
@@ -1272,7 +1350,7 @@ extern "C" int lbm_set_spacing(lbm_solver* h, const double* dx, const double* dy
   for (int y = 0; y < s.ny; y++) r[s.nx + y] = s.dt / s.dys[y];
   for (int x = 0; x < s.nx; x++) r[s.nx + s.ny + x] = s.dxs[x];
   for (int y = 0; y < s.ny; y++) r[2 * s.nx + s.ny + y] = s.dys[y];
-  if (!s.spacing_dev) HIP_TRY(hipMalloc((void**)&s.spacing_dev, r.size() * sizeof(double)));
+  if (!s.spacing_dev) HIP_TRY(solver_alloc(s, (void**)&s.spacing_dev, r.size() * sizeof(double)));
   HIP_TRY(hipMemcpy(s.spacing_dev, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
   return upload_inst(s);
 }

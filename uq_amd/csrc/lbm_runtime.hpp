@@ -88,6 +88,7 @@ struct Solver {
   double* res_hist = nullptr;       // served with a tolerance: per-step residual history (kResHist)
   int* fails_dev = nullptr;         // failure step gathered for the host (one int)
   std::unique_ptr<GraphCache> graphs{new GraphCache()};
+  std::vector<std::pair<void*, size_t>> owned;  // buffers from solver_alloc, returned to its cache
   int ntiles = 0, nwalls = 0, nptiles = 0;
   int nphys = 0;  // flux tiles flagged kTilePhysical
   int nstrips = 0, nstrips_fast = 0;
@@ -162,6 +163,10 @@ struct Batch {
 };
 
 int fail_with(int status, const char* fmt, ...);
+// A device buffer of `bytes` for solver s: a kept buffer of that size from a
+// destroyed solver on this device when there is one (contents undefined),
+// else hipMalloc; ~Solver returns it.
+hipError_t solver_alloc(Solver& s, void** p, size_t bytes);
 // Strip flux kernels in use (lbm_set_strips; default on), and whether a
 // launch group with `nstrips` strip items takes them.
 bool strips_enabled();

@@ -200,7 +200,7 @@ static int admit(Server& sv) {
     ActiveReq a{s, r.nsteps, r.tag, r.submit_s, t, s->device_bytes};
     a.tol = r.params.tol > 0 ? r.params.tol : 0.0;
     if (a.tol > 0) {
-      SV_TRY(hipMalloc((void**)&s->res_hist, kResHist * sizeof(double)));
+      SV_TRY(solver_alloc(*s, (void**)&s->res_hist, kResHist * sizeof(double)));
       SV_TRY(hipMemset(s->res_hist, 0, kResHist * sizeof(double)));
     }
     sv.active.push_back(a);

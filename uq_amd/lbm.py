@@ -60,6 +60,8 @@ def lib():
     L.lbm_set_kernel_timing.argtypes = [i32]
     L.lbm_set_graphs.argtypes = [i32]
     L.lbm_set_strips.argtypes = [i32]
+    L.lbm_release_cached.argtypes = []
+    L.lbm_release_cached.restype = None
     L.lbm_get_totals.argtypes = [vp, P(d)]
     L.lbm_set_flux_shape.argtypes = [i32, i32]
     L.lbm_set_viscosity.argtypes = [vp, i32, d, d, d]
@@ -124,6 +126,11 @@ def set_strips(mode) -> None:
     the GPU, 2 strips wherever possible."""
     mode = int(mode)
     _check(lib().lbm_set_strips(mode), "lbm_set_strips")
+
+
+def release_cached() -> None:
+    """Return the device buffers kept from destroyed solvers to the device."""
+    lib().lbm_release_cached()
 
 
 def last_timing() -> dict:
