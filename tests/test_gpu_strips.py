@@ -7,7 +7,8 @@ doubly periodic (shear wave), periodic with solid walls (Couette, WENO5 and
 NND), clamped edges (Riemann), a north/south mirror with a clamp opposite
 and a solid block (obstacle), east/west mirrors, a stretched grid
 (per-column dt/dx, per-row dt/dy), the one-stage steppers (RK1, Euler), grids
-whose sides are no multiple of the strip width or height, and mixed
+whose sides are no multiple of the strip width or height, instances
+narrower than one strip and shorter than one row block, and mixed
 precision (FP32 storage: the stage-2 flux runs on strips, the
 moment-summing stage-3 flux stays on tiles).
 
@@ -44,6 +45,11 @@ DECKS = {
     "shear_stretched": lambda: cases.stretched(cases.shear_wave(80, u0=0.15, steps=30), amp=0.3),
     "shear_rk1": lambda: cases.shear_wave(64, u0=0.1, rk_method=0, steps=30),
     "shear_euler": lambda: cases.shear_wave(64, u0=0.1, rk_method=2, steps=30),
+    # instances narrower than one strip and shorter than one row block: lanes
+    # beyond the last column and rows beyond the last row map (and wrap) far
+    # past the edges
+    "tiny_couette": lambda: cases.couette(9, nx=13, steps=10),
+    "tiny_shear": lambda: cases.shear_wave(12, u0=0.1, steps=10),
 }
 
 
