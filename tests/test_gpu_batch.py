@@ -2,7 +2,10 @@
 
 Per-instance arithmetic does not depend on the batch: every member of a
 batch, and every request served in a mix, must equal the same solve run
-alone bit for bit.
+alone bit for bit — also when the group runs its flux stages on the strip
+kernels and the lone run on the tile kernels (lbm_set_strips 2 and 0; the
+default rule gives these small groups tiles), a group that mixes strip
+instances with tile instances (the plate needs the physical path).
 """
 import numpy as np
 import pytest
@@ -24,25 +27,35 @@ def _small_ensemble():
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_equals_alone_bitwise(gpu_lib, precision):
+# (precision, strip mode of the batch, strip mode of the lone runs): the
+# mixed ensemble holds a shock deck whose strip stencils are FP64 and tile
+# stencils FP32, so its cross-kernel case compares strips with strips
+@pytest.mark.parametrize("precision,batch_mode,alone_mode",
+                         [("fp64", 1, 1), ("mixed", 1, 1), ("fp64", 2, 0), ("mixed", 2, 2)],
+                         ids=["fp64", "mixed", "fp64_strips_vs_tiles", "mixed_strips"])
+def test_batch_equals_alone_bitwise(gpu_lib, precision, batch_mode, alone_mode):
     lbm = gpu_lib
     ens = _small_ensemble()
-    print(f"[batch] {len(ens)} members precision={precision}", flush=True)
-    alone = []
-    for c in ens:
-        s = lbm.Solver(c, precision)
-        assert s.step(30) == 0
-        alone.append(s.get_dist())
-        s.close()
-    solvers = [lbm.Solver(c, precision) for c in ens]
-    b = lbm.Batch(solvers)
-    assert b.step(12) == 0
-    assert b.step(18) == 0
-    for s, (fa, ha) in zip(solvers, alone):
-        f, h = s.get_dist()
-        assert s.info()["step"] == 30
-        assert np.array_equal(f, fa) and np.array_equal(h, ha)
+    print(f"[batch] {len(ens)} members precision={precision} modes {batch_mode}/{alone_mode}", flush=True)
+    try:
+        lbm.set_strips(alone_mode)
+        alone = []
+        for c in ens:
+            s = lbm.Solver(c, precision)
+            assert s.step(30) == 0
+            alone.append(s.get_dist())
+            s.close()
+        lbm.set_strips(batch_mode)
+        solvers = [lbm.Solver(c, precision) for c in ens]
+        b = lbm.Batch(solvers)
+        assert b.step(12) == 0
+        assert b.step(18) == 0
+        for s, (fa, ha) in zip(solvers, alone):
+            f, h = s.get_dist()
+            assert s.info()["step"] == 30
+            assert np.array_equal(f, fa) and np.array_equal(h, ha)
+    finally:
+        lbm.set_strips(1)
 
 
 def test_batch_member_failure_is_isolated(gpu_lib):
@@ -67,8 +80,17 @@ def test_batch_member_failure_is_isolated(gpu_lib):
     assert s_bad.step(1) == 1
 
 
-def test_server_results_equal_alone(gpu_lib):
+@pytest.mark.parametrize("server_mode,alone_mode", [(1, 1), (2, 0)], ids=["default", "strips_vs_tiles"])
+def test_server_results_equal_alone(gpu_lib, server_mode, alone_mode):
     lbm = gpu_lib
+    lbm.set_strips(server_mode)
+    try:
+        _serve_and_compare(lbm, alone_mode)
+    finally:
+        lbm.set_strips(1)
+
+
+def _serve_and_compare(lbm, alone_mode):
     rng = np.random.default_rng(5)
     reqs = []
     for k in range(7):
@@ -91,6 +113,7 @@ def test_server_results_equal_alone(gpu_lib):
         if not active and not queued:
             break
     assert sorted(r["tag"] for r in results) == list(range(len(reqs)))
+    lbm.set_strips(alone_mode)
     for r in results:
         c = reqs[r["tag"]]
         assert r["status"] == 0 and r["steps_done"] == c.nsteps
