@@ -147,6 +147,19 @@ def test_flux_shape_table():
         lbm.flux_shape_dims(lbm.FLUX_SHAPES)
 
 
+def test_strip_mode_switch():
+    """lbm_set_strips takes modes 0 (tiles), 1 (size rule, the default) and
+    2 (strips wherever possible) and refuses others (host setting, no GPU)."""
+    try:
+        for mode in (0, 2, 1):
+            lbm.set_strips(mode)
+        for bad in (-1, 3):
+            with pytest.raises(lbm.LbmError):
+                lbm.set_strips(bad)
+    finally:
+        lbm.set_strips(1)
+
+
 def test_integration_binding_compiles(tmp_path):
     """The maintainer-side binding published with the repository (the
     `gpuStep.h` block of the integration notes) compiles against
