@@ -55,17 +55,23 @@ def log(msg: str) -> None:
 
 def kernel_bytes_per_lup(es: int) -> dict:
     """Compulsory HBM traffic of each kernel per lattice update, for values
-    of `es` bytes. FP64 storage: s12 reads the state and writes f2 (52
-    values); flux2 reads f2 and writes the 24 net-flux planes (50); s3
-    reads the state and the net fluxes (summing the net fluxes' moments in
-    a first pass whose lines the second pass finds in L2) and writes f3 and
-    G (102); flux3 reads f3 and G and writes the state (78). FP32 storage
-    adds the 4 FP64 conserved-moment planes of the state: read by s12 and
-    s3, written by s3 (those of G) and read and written by flux3. Halo
-    re-reads, the cell-type map and the wall-closure kernels are
-    excluded."""
-    mom = 32 if es == 4 else 0
-    return {"s12": 52 * es + mom, "flux2": 50 * es, "s3": 102 * es + 2 * mom, "flux3": 78 * es + 2 * mom}
+    of `es` bytes, counting the planes each kernel touches (13 f and 13 h
+    directions; the rest direction has no flux). s12 reads the state and
+    writes f2 but for its rest direction (26 + 24 values); flux2 reads the
+    24 moving directions of f2 and writes the 24 net-flux planes (48). FP64
+    storage: s3 reads the state and the net fluxes (summing the net fluxes'
+    moments in a first pass whose lines the second pass finds in L2) and
+    writes f3, its rest direction straight into the new state (26 + 24 +
+    26); flux3 reads f3 and the stage-2 net fluxes and writes the new state
+    f3 + (dF2 - dF3)/2 (24 + 24 + 24). FP32 storage: s3 writes f3 and
+    G = f3 + dF2/2 (26 + 24 + 52) and flux3 reads f3 and G (24 + 24 + 24),
+    and the 4 FP64 conserved-moment planes of the state add 32 B to s12
+    (read), 64 B to s3 (read, written for G) and 64 B to flux3 (read and
+    written). Halo re-reads, the cell-type map and the wall-closure kernels
+    are excluded."""
+    if es == 4:
+        return {"s12": 50 * es + 32, "flux2": 48 * es, "s3": 102 * es + 64, "flux3": 72 * es + 64}
+    return {"s12": 50 * es, "flux2": 48 * es, "s3": 76 * es, "flux3": 72 * es}
 
 
 def measured_traffic(kernel: str, workload: str, shape: int):

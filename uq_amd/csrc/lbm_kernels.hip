@@ -11,13 +11,16 @@
 //              fluxes of f2 (wallFluxes, cpuLBM.cpp:1502-1590)]
 //   k_flux<0>  stencil: per direction, f2 of tile + 3-cell halo in LDS,
 //              x/y face fluxes (WENO5 or NND), net flux dF(f2) -> dF planes
-//              and the moment sums of dF (posFluxes + cFLUX,
-//              cpuLBM.cpp:1436-1500)
+//              (posFluxes + cFLUX, cpuLBM.cpp:1436-1500)
 //   k_s3       pointwise: moments of f - dF(f2), stage-3 equilibrium,
-//              implicit stage 3 -> f3, and the final combination except its
-//              last flux term -> the other state buffer (cpuLBM.cpp:1926-2042)
+//              implicit stage 3 -> f3 (cpuLBM.cpp:1926-1993)
 //  [k_alpha    same for f3]
-//   k_flux<1>  stencil: net flux dF(f3); new state -= dF(f3)/2
+//   k_flux<1>  stencil: net flux dF(f3); the final combination, new state
+//              f3 + (dF(f2) - dF(f3))/2 (cpuLBM.cpp:2009-2042)
+//
+// On launch groups large enough to fill the GPU the stencil stages run on
+// the strip kernels of lbm_strip.hpp (one wavefront per 58-column strip,
+// rows marched through registers) instead of the tile kernels below.
 //
 // The stencil kernels run one direction at a time through a runtime loop
 // (one copy of the flux code, low register use) on tiles of one of five
@@ -359,10 +362,14 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const
     TC fe, he, f2, h2;
     eq_stored<I, TS, true, TE>(in, r.q, qd, fe, he, sumfe, sumabs);
     s12<I>(r, hdt, TC(fval<I>(cs)), TC(hval<I>(cs)), fe, he, f2, h2);
-    // streaming stores: f2 and, below, f3 and G are written once per step
-    // and read back only by the next kernel, long after L2 has turned over
-    __builtin_nontemporal_store(TS(f2), &dst[P * I + o]);
-    __builtin_nontemporal_store(TS(h2), &dst[P * (kQ + I) + o]);
+    // streaming stores: f2 and, in stage 3, f3 are written once per step
+    // and read back only by the next kernel, long after L2 has turned over.
+    // The rest direction of f2 has no flux and no wall term, and stage 3
+    // recomputes f2 from the state: nothing reads it.
+    if constexpr (I != 0) {
+      __builtin_nontemporal_store(TS(f2), &dst[P * I + o]);
+      __builtin_nontemporal_store(TS(h2), &dst[P * (kQ + I) + o]);
+    }
   };
   (one(std::integral_constant<int, Is>{}), ...);
   flag_failure(in, step, m.rho, failure_mass<TS>(in, sumfe), eq_check_tol<TE>(sumabs));
@@ -392,15 +399,22 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
 }
 
 // ---------------------------------------------------------------------------
-// k_s3: RK3 stage 3 and the final combination up to its dF(f3) term
-// (cpuLBM.cpp:1926-1993, 2009-2040), pointwise, one pass over the
-// directions. With P = f - dF2 +
-// af2 (feq2 - f2) and Q = (h - dH2) - dt/2 A2 + ah2 (heq2 - h2):
-//   f3 = (P + af3 feq3)/(1 + af3),  h3 = (Q - dt/2 A3 + ah3 heq3)/(1 + ah3)
-//   G  = P + dF2/2 + dt/2 (feq3 - f3)/tf3                       (f part)
-//   G  = Q + dH2/2 + dt/2 (heq3 - h3)/th3 - dt/2 A3              (h part)
-// and the new state is G - dF(f3)/2; A = (e.u/thf)(feq - f). (P + dF2/2 is
-// f - dF2/2 + dt/2 (feq2 - f2)/tf2 since af2 = dt/(2 tf2).)
+// k_s3: RK3 stage 3 (cpuLBM.cpp:1926-1993), pointwise, one pass over the
+// directions. With P = f - dF2 + af2 (feq2 - f2) and
+// Q = (h - dH2) - dt/2 A2 + ah2 (heq2 - h2), A = (e.u/thf)(feq - f):
+//   f3 = (P + af3 feq3)/(1 + af3),  h3 = (Q - dt/2 A3 + ah3 heq3)/(1 + ah3).
+// The final combination (cpuLBM.cpp:2009-2040) is
+//   f' = P + dF2/2 + dt/2 (feq3 - f3)/tf3 - dF3/2
+//   h' = Q + dH2/2 + dt/2 (heq3 - h3)/th3 - dt/2 A3 - dH3/2
+// (P + dF2/2 is f - dF2/2 + dt/2 (feq2 - f2)/tf2 since af2 = dt/(2 tf2)),
+// and since f3 = P + af3 (feq3 - f3) and h3 = Q - dt/2 A3 + ah3 (heq3 - h3)
+// with af3 = dt/(2 tf3), ah3 = dt/(2 th3), that is
+//   f' = f3 + (dF2 - dF3)/2,  h' = h3 + (dH2 - dH3)/2:
+// FP64 storage: the stage-3 flux kernel forms the new state from f3 and the
+// stored dF2, and this kernel writes only f3 (its rest direction, which has
+// no flux, straight into the new state). FP32 storage (kStage3G): this
+// kernel writes f3 and G = f' + dF3/2, formed here in FP64 as above, and
+// the flux kernel subtracts dF3/2.
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, typename TE, int... Is>
 __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb,
@@ -427,7 +441,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
   finish_macro(in, s[0], s[1], s[2], s[3], m1);
   finish_macro(in, s[0] - drho, s[1] - drux, s[2] - druy, s[3] - dsh, m3);
   if constexpr (sizeof(TS) == 4) {
-    // conserved moments of G = the new state before its dF(f3)/2 term: the
+    // conserved moments of the new state before its dF(f3)/2 term: the
     // relaxation terms move none, so they are those of f - dF(f2)/2; the
     // stage-3 flux kernel subtracts half those of dF(f3)
     mom_out[o] = s[0] - 0.5 * drho;
@@ -467,12 +481,18 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
     const TC eu3 = TC(lat_x(I)) * cU3 + TC(lat_y(I)) * cV3;
     const TC A3 = (eu3 * ithf3) * (fe3 - f3);
     const TC h3 = ((Qh - hdt * A3) + ah3 * he3) * iah3;
-    const TC gf = (Pf + TC(0.5) * dF) + af3 * (fe3 - f3);
-    const TC gh = ((Qh + TC(0.5) * dH) + ah3 * (he3 - h3)) - hdt * A3;
-    __builtin_nontemporal_store(TS(f3), &f3b[P * I + o]);
-    __builtin_nontemporal_store(TS(h3), &f3b[P * (kQ + I) + o]);
-    __builtin_nontemporal_store(TS(gf), &out[P * I + o]);
-    __builtin_nontemporal_store(TS(gh), &out[P * (kQ + I) + o]);
+    if constexpr (kStage3G<TS>) {
+      const TC gf = (Pf + TC(0.5) * dF) + af3 * (fe3 - f3);
+      const TC gh = ((Qh + TC(0.5) * dH) + ah3 * (he3 - h3)) - hdt * A3;
+      __builtin_nontemporal_store(TS(f3), &f3b[P * I + o]);
+      __builtin_nontemporal_store(TS(h3), &f3b[P * (kQ + I) + o]);
+      __builtin_nontemporal_store(TS(gf), &out[P * I + o]);
+      __builtin_nontemporal_store(TS(gh), &out[P * (kQ + I) + o]);
+    } else {
+      TS* dst = I == 0 ? out : f3b;
+      __builtin_nontemporal_store(TS(f3), &dst[P * I + o]);
+      __builtin_nontemporal_store(TS(h3), &dst[P * (kQ + I) + o]);
+    }
     // one direction at a time: keeps the scheduler from interleaving the
     // directions' temporaries (FP32 storage: 38 spilled registers otherwise)
     if constexpr (sizeof(TS) == 4) __builtin_amdgcn_sched_barrier(0);
@@ -1039,11 +1059,13 @@ __device__ __forceinline__ void face_flux(const FH<TC>* V, const Region<Sh>& R, 
 // ---------------------------------------------------------------------------
 // k_flux<MODE>: net flux of every direction for the tile's cells.
 //   MODE 0: source f2 (f1 for RK1); dF, dH -> the dF planes.
-//   MODE 1: source f3; new state (holding G) -= dF/2, dH/2.
+//   MODE 1: source f3; new state f3 + (dF2 - dF)/2, h3 + (dH2 - dH)/2
+//           from the stored stage-2 net fluxes dF2, dH2 (FP32 storage:
+//           from G, which stage 3 writes, G - dF/2, G - dH/2).
 //   MODE 2: source the current state (EULER, EULER_EQ); dF, dH -> dF planes.
-// MOM (MODE 1, FP32 storage): also the new state's conserved moments,
-// those of G minus half those of each fluid cell's dF(f3) over the
-// directions -> the moment planes of the new state buffer.
+// MOM (MODE 1, FP32 storage): also the new state's conserved moments, those
+// stage 3 wrote (of f - dF2/2) minus half those of each fluid cell's dF(f3)
+// over the directions -> the moment planes of the new state buffer.
 //
 // FP32 storage: directions alternate between two staging buffers filled by
 // direct-to-LDS loads; while the threads compute direction I's faces from
@@ -1203,6 +1225,7 @@ struct TileWork {
   int method;
   const void* src;
   TS* dst;
+  const TS* dfl;  // MODE 1: the stored stage-2 net fluxes
   const double* alpha;
   long long P;
   double c;
@@ -1258,20 +1281,33 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
     face_flux<1>(cur.v, R, alpha, method, W.solids, col, j - (sy > 0 ? 1 : 0), sy, v, rf, rh, vf, vh);
     S.fy[j][col] = FH<TC>{vf, vh};
   }
+  // MODE 1 (FP64 storage): the own cells' f3 of direction I, taken from cur
+  // before the barrier after which (one buffer) the next direction may
+  // overwrite it
+  FH<TC> own[Sh::Cells];
+  if constexpr (MODE == 1 && !kStage3G<TS>) {
+#pragma unroll
+    for (int q = 0; q < Sh::Cells; q++) {
+      const int ci = t + q * Sh::NT;
+      own[q] = W.fluid[q] ? cur.v[(ci / Sh::TX + kH) * Sh::RX + ci % Sh::TX + kH] : FH<TC>{TC(0), TC(0)};
+    }
+  }
   if constexpr (DB) __builtin_amdgcn_s_waitcnt(kWaitVmem);  // this wave's loads into nxt landed
   __syncthreads();
   // One buffer without the moment sums: the loads of direction I+1 go out
   // now and land in registers while the cells update (written to LDS when
   // I+1 starts; stage-3 flux 5-6 % faster, while the moment-summing kernel,
   // short of registers, loads at the start of each direction instead).
-  // MODE 1 first issues the loads of the values it updates, which it needs
-  // sooner.
-  TS oldf[Sh::Cells], oldh[Sh::Cells];
+  // MODE 1 first issues the loads of the stored dF(f2) (FP32 storage: of
+  // G), which it needs sooner.
+  TS d2f[Sh::Cells], d2h[Sh::Cells];
   if constexpr (MODE == 1) {
+    const TS* b = kStage3G<TS> ? W.dst : W.dfl;
+    const long long pf = kStage3G<TS> ? P * I : P * (I - 1), ph = kStage3G<TS> ? P * (kQ + I) : P * (12 + I - 1);
 #pragma unroll
     for (int q = 0; q < Sh::Cells; q++) {
-      oldf[q] = W.fluid[q] ? W.dst[P * I + W.oo[q]] : TS(0);
-      oldh[q] = W.fluid[q] ? W.dst[P * (kQ + I) + W.oo[q]] : TS(0);
+      d2f[q] = W.fluid[q] ? b[pf + W.oo[q]] : TS(0);
+      d2h[q] = W.fluid[q] ? b[ph + W.oo[q]] : TS(0);
     }
   }
   if constexpr (!DB && !MOM) {
@@ -1350,8 +1386,15 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
         S.mom[3][ci] += double(TS(dH));
       }
     } else {
-      W.dst[P * I + o] = TS(TC(oldf[q]) - TC(0.5) * dF);
-      W.dst[P * (kQ + I) + o] = TS(TC(oldh[q]) - TC(0.5) * dH);
+      // the new state f3 + (dF2 - dF3)/2 (FP32 storage: G - dF3/2), the
+      // final combination of k_s3
+      if constexpr (kStage3G<TS>) {
+        W.dst[P * I + o] = TS(TC(d2f[q]) - TC(0.5) * dF);
+        W.dst[P * (kQ + I) + o] = TS(TC(d2h[q]) - TC(0.5) * dH);
+      } else {
+        W.dst[P * I + o] = TS(own[q].f + TC(0.5) * (TC(d2f[q]) - dF));
+        W.dst[P * (kQ + I) + o] = TS(own[q].h + TC(0.5) * (TC(d2h[q]) - dH));
+      }
       if constexpr (MOM) {  // conserved moments of dF(f3), for the FP64 moment planes
         const double dFs = double(dF);
         S.mom[0][ci] += dFs;
@@ -1389,6 +1432,7 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
   W.P = in.plane;
   W.src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
   W.dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
+  W.dfl = reinterpret_cast<const TS*>(in.dflux);
   // own cells of this thread: cell ci = t + q NT at (ci % TX, ci / TX)
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
@@ -1416,8 +1460,8 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
     for (int I = 1; I < kQ; I++) flux_dir<TS, TC, MODE, MOM, false>(S, R, in, W, L, PV, VA, VA, I, I + 1 < kQ);
   }
   if constexpr (MOM) {
-    // the new state's conserved moments: those of G (written by stage 3)
-    // minus half those of dF(f3)
+    // the new state's conserved moments: those written by stage 3 (of
+    // f - dF(f2)/2) minus half those of dF(f3)
     static_assert(MODE == 1, "moment sums belong to the stage-3 flux of FP32 storage");
     double* mo = in.mom[1 ^ in.parity0 ^ par];
 #pragma unroll
@@ -1454,8 +1498,10 @@ __device__ __forceinline__ void flux_tiles(const InstDev* insts, const TileRef* 
 // wrong: each thread takes an own fluid cell and forms its twelve net
 // fluxes with the reference's addressing (phys_net), then writes what the
 // flux kernel writes for the cell (MODE 0 and 2: the dF planes; MODE 1: the
-// new state G - dF/2, with MOM the moment planes' update). Launched after the flux kernel, only for groups with
-// such tiles; the flux kernel skips them.
+// new state f3 + (dF2 - dF3)/2, FP32 storage G - dF3/2, with MOM the moment
+// planes' update).
+// Launched after the flux kernel, only for groups with such tiles; the flux
+// kernel skips them.
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 __global__ __launch_bounds__(256) void k_phys(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
                                               const int* stepp) {
@@ -1487,8 +1533,14 @@ __global__ __launch_bounds__(256) void k_phys(const InstDev* insts, const TileRe
           vf = double(TS(dF));
           vh = double(TS(dH));
         } else {
-          dst[P * I + o] = TS(TC(dst[P * I + o]) - TC(0.5) * dF);
-          dst[P * (kQ + I) + o] = TS(TC(dst[P * (kQ + I) + o]) - TC(0.5) * dH);
+          if constexpr (kStage3G<TS>) {
+            dst[P * I + o] = TS(TC(dst[P * I + o]) - TC(0.5) * dF);
+            dst[P * (kQ + I) + o] = TS(TC(dst[P * (kQ + I) + o]) - TC(0.5) * dH);
+          } else {
+            const TS* d2 = reinterpret_cast<const TS*>(in.dflux);
+            dst[P * I + o] = TS(TC(src[P * I + o]) + TC(0.5) * (TC(d2[P * (I - 1) + o]) - dF));
+            dst[P * (kQ + I) + o] = TS(TC(src[P * (kQ + I) + o]) + TC(0.5) * (TC(d2[P * (12 + I - 1) + o]) - dH));
+          }
           vf = double(dF);
           vh = double(dH);
         }

@@ -132,6 +132,7 @@ template <typename TS>
 struct StripInst {
   gptr<const TS> src;
   gptr<TS> dst;
+  gptr<const TS> dfl;  // MODE 1: the stored stage-2 net fluxes
   gptr<const uint8_t> ct;
   gptr<const double> alpha;
   gptr<const double> rdyp;  // dt/dy per row (rdy_stride 1) or the instance's one value (0)
@@ -163,9 +164,13 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   const gptr<const uint8_t> ct = in.ct + C.X;
   const gptr<const double> al = in.alpha + C.X;
   // this lane's own cell in the planes it writes (MODE 0, 2: the dF planes
-  // of direction I; MODE 1: the state planes) and reads (MODE 1)
+  // of direction I; MODE 1: the state planes) and, MODE 1, in the planes of
+  // direction I it reads: the stored dF(f2) (FP32 storage: G, in the state
+  // planes it writes)
   const gptr<TS> of = in.dst + (MODE == 1 ? (long long)I : (long long)(I - 1)) * in.P + C.X;
   const gptr<TS> oh = in.dst + (MODE == 1 ? (long long)(kQ + I) : (long long)(12 + I - 1)) * in.P + C.X;
+  const gptr<const TS> df2 = kStage3G<TS> ? (gptr<const TS>)of : in.dfl + (long long)(I - 1) * in.P + C.X;
+  const gptr<const TS> dh2 = kStage3G<TS> ? (gptr<const TS>)oh : in.dfl + (long long)(12 + I - 1) * in.P + C.X;
 
   auto map_row = [&](int yl, unsigned& my) {
     if (in.inner) {
@@ -263,7 +268,8 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   // Everything a row needs from memory, loaded one row ahead: the staged
   // position entering the window (row y + 3; row y itself without a y
   // component), the own cell's type as loaded (0 fluid; 1 for lanes without
-  // an own cell), its row offset, (MODE 1) the values it updates, and dt/dy
+  // an own cell), its row offset, (MODE 1) its stored dF(f2) (FP32
+  // storage: G), and dt/dy
   // of the row (on a uniform grid the instance's own value, read with
   // stride 0, so that every row issues the same loads).
   struct Pre {
@@ -281,8 +287,8 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
     if (C.own) {
       r.type = ct[r.ro];
       if (MODE == 1) {
-        r.f = of[r.ro];
-        r.h = oh[r.ro];
+        r.f = df2[r.ro];
+        r.h = dh2[r.ro];
       }
     }
     r.rdy = TC(in.rdyp[y * in.rdy_stride]);
@@ -366,8 +372,16 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
       r.vf = TS(dF);
       r.vh = TS(dH);
     } else {
-      r.vf = TS(TC(cc.f) - TC(0.5) * dF);
-      r.vh = TS(TC(cc.h) - TC(0.5) * dH);
+      // the new state f3 + (dF2 - dF3)/2 (FP32 storage: G - dF3/2), the
+      // final combination of the stage-3 kernel; the own cell's f3 is the
+      // staged value of row y
+      if constexpr (kStage3G<TS>) {
+        r.vf = TS(TC(cc.f) - TC(0.5) * dF);
+        r.vh = TS(TC(cc.h) - TC(0.5) * dH);
+      } else {
+        r.vf = TS(xr.f + TC(0.5) * (TC(cc.f) - dF));
+        r.vh = TS(xr.h + TC(0.5) * (TC(cc.h) - dH));
+      }
     }
     return r;
   };
@@ -435,6 +449,7 @@ __device__ __forceinline__ void strip_item(const InstDev& in, const TileRef& sr,
   StripInst<TS> si;
   si.src = (gptr<const TS>)(MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]));
   si.dst = (gptr<TS>)(MODE == 1 ? in.state[1 ^ in.parity0 ^ par] : in.dflux);
+  si.dfl = (gptr<const TS>)in.dflux;
   si.ct = (gptr<const uint8_t>)in.ctype;
   si.alpha = (gptr<const double>)in.alpha;
   si.rdyp = (gptr<const double>)(in.rdys ? in.rdys : &in.rdy);
