@@ -21,14 +21,16 @@
 
 // DPP controls of whole-wave shifts by one lane (GFX9 wave_shr:1 and
 // wave_shl:1): lane l takes the value of lane l - 1, respectively l + 1;
-// the lane shifted in at the end keeps 0. Every shift runs with all 64
-// lanes active (never inside a divergent branch).
+// the lane shifted in at the end reads 0 (bound_ctrl: the zero comes from
+// the instruction, so no register has to be cleared before each shift, as
+// an "old" operand of 0 would need). Every shift runs with all 64 lanes
+// active (never inside a divergent branch).
 constexpr int kDppShr1 = 0x138;
 constexpr int kDppShl1 = 0x130;
 
 template <int CTRL>
 __device__ __forceinline__ int lane_shift(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float lane_shift(float v) {
