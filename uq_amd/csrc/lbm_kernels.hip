@@ -121,6 +121,23 @@ __device__ __forceinline__ double eq_check_tol(double sum_abs) {
   return std::is_same<TE, double>::value ? 1e-6 : 1e-6 + 16.0 * 5.96e-8 * sum_abs;
 }
 
+// Global-memory pointers: loads and stores through them are global, not
+// flat, operations (the pointers read out of the instance table are
+// generic), so they count on the vector-memory counter only: a flat access
+// also counts on the LDS/scalar counter, and every wait for a scalar load
+// then waits for it too.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(gptr<T> p) {
+  return p;
+}
+
 // ---------------------------------------------------------------------------
 // Pointwise helpers
 // ---------------------------------------------------------------------------
@@ -165,10 +182,10 @@ struct CellState {
   double cf[5], ch;  // FP32 storage: corrections of f0..f4 and h0
 };
 
-template <typename TS>
-__device__ __forceinline__ void load_state(const InstDev& in, const TS* src, const double* mom, long long o,
-                                           CellState<TS>& c) {
+template <typename TS, typename PS, typename PM>
+__device__ __forceinline__ void load_state(const InstDev& in, PS srcp, PM momp, long long o, CellState<TS>& c) {
   const long long P = in.plane;
+  const gptr<const TS> src = as_global(srcp);
 #pragma unroll
   for (int i = 0; i < kQ; i++) {
     c.f[i] = __builtin_nontemporal_load(&src[P * i + o]);
@@ -186,6 +203,7 @@ __device__ __forceinline__ void load_state(const InstDev& in, const TS* src, con
   if constexpr (sizeof(TS) == 4) {
 #pragma unroll
     for (int k = 0; k < 4; k++) t[k] += in.refsum[k];
+    const gptr<const double> mom = as_global(momp);
 #pragma unroll
     for (int k = 0; k < 4; k++) c.s[k] = __builtin_nontemporal_load(&mom[P * k + o]);
     const double dx = (c.s[1] - t[1]) * in.i2c, dy = (c.s[2] - t[2]) * in.i2c;
@@ -196,7 +214,7 @@ __device__ __forceinline__ void load_state(const InstDev& in, const TS* src, con
     c.cf[4] = -dy;
     c.ch = c.s[3] - t[3];
   } else {
-    (void)mom;
+    (void)momp;
 #pragma unroll
     for (int k = 0; k < 4; k++) c.s[k] = t[k];
   }
@@ -346,8 +364,8 @@ __device__ __forceinline__ bool eq_kernel_for(const InstDev& in) {
 // k_s12: RK3 stages 1 and 2, pointwise -> f2.
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, typename TE, int... Is>
-__device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const double* mom, TS* dst, long long o,
-                                         int step, Seq<Is...>) {
+__device__ __forceinline__ void s12_cell(const InstDev& in, gptr<const TS> src, gptr<const double> mom, gptr<TS> dst,
+                                         long long o, int step, Seq<Is...>) {
   const long long P = in.plane;
   CellState<TS> cs;
   load_state(in, src, mom, o, cs);
@@ -376,8 +394,8 @@ __device__ __forceinline__ void s12_cell(const InstDev& in, const TS* src, const
 }
 
 template <typename TS, typename TC, typename TE>
-__global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                             const int* stepp) {
+__global__ __launch_bounds__(kNT) void k_s12(const InstDev* __restrict__ insts, const TileRef* __restrict__ tiles,
+                                             int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
@@ -387,14 +405,14 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par;
-    const TS* src = state_buf<TS>(in, cur);
-    TS* dst = reinterpret_cast<TS*>(in.f2);
-    if (in.ctype[c.o] > 0) {
+    const gptr<const TS> src = as_global(state_buf<TS>(in, cur));
+    const gptr<TS> dst = as_global(reinterpret_cast<TS*>(in.f2));
+    if (as_global(in.ctype)[c.o] > 0) {
 #pragma unroll
       for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
       continue;
     }
-    s12_cell<TS, TC, TE>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
+    s12_cell<TS, TC, TE>(in, src, as_global(in.mom[cur]), dst, c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
@@ -417,8 +435,9 @@ __global__ __launch_bounds__(kNT) void k_s12(const InstDev* insts, const TileRef
 // the flux kernel subtracts dF3/2.
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, typename TE, int... Is>
-__device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb,
-                                        TS* f3b, TS* out, double* mom_out, long long o, int step, Seq<Is...>) {
+__device__ __forceinline__ void s3_cell(const InstDev& in, gptr<const TS> src, gptr<const double> mom,
+                                        gptr<const TS> dfb, gptr<TS> f3b, gptr<TS> out, gptr<double> mom_out, long long o,
+                                        int step, Seq<Is...>) {
   const long long P = in.plane;
   CellState<TS> cs;
   load_state(in, src, mom, o, cs);
@@ -489,7 +508,7 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
       __builtin_nontemporal_store(TS(gf), &out[P * I + o]);
       __builtin_nontemporal_store(TS(gh), &out[P * (kQ + I) + o]);
     } else {
-      TS* dst = I == 0 ? out : f3b;
+      const gptr<TS> dst = I == 0 ? out : f3b;
       __builtin_nontemporal_store(TS(f3), &dst[P * I + o]);
       __builtin_nontemporal_store(TS(h3), &dst[P * (kQ + I) + o]);
     }
@@ -504,13 +523,15 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, const TS* src, const 
 // Frozen cell (permanent or solid) of a pointwise stage: the stored values
 // (and, for FP32 storage, the moment planes) carried over unchanged.
 template <typename TS>
-__device__ __forceinline__ void copy_frozen(const InstDev& in, const TS* src, TS* dst, int from, int to,
+__device__ __forceinline__ void copy_frozen(const InstDev& in, gptr<const TS> src, gptr<TS> dst, int from, int to,
                                             long long o) {
 #pragma unroll
   for (int k = 0; k < kPlanes; k++) dst[in.plane * k + o] = src[in.plane * k + o];
   if constexpr (sizeof(TS) == 4) {
+    const gptr<const double> mf = as_global(in.mom[from]);
+    const gptr<double> mt = as_global(in.mom[to]);
 #pragma unroll
-    for (int k = 0; k < 4; k++) in.mom[to][in.plane * k + o] = in.mom[from][in.plane * k + o];
+    for (int k = 0; k < 4; k++) mt[in.plane * k + o] = mf[in.plane * k + o];
   } else {
     (void)from;
     (void)to;
@@ -523,7 +544,7 @@ __device__ __forceinline__ void copy_frozen(const InstDev& in, const TS* src, TS
 // FP64: 5.9 -> 4.2 ms per launch).
 template <typename TS, typename TC, typename TE>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k_s3(
-    const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
+    const InstDev* __restrict__ insts, const TileRef* __restrict__ tiles, int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
@@ -533,18 +554,18 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
-    const TS* src = state_buf<TS>(in, cur);
-    TS* out = reinterpret_cast<TS*>(in.state[nxt]);
-    TS* f3b = reinterpret_cast<TS*>(in.f3);
-    if (in.ctype[c.o] > 0) {
+    const gptr<const TS> src = as_global(state_buf<TS>(in, cur));
+    const gptr<TS> out = as_global(reinterpret_cast<TS*>(in.state[nxt]));
+    const gptr<TS> f3b = as_global(reinterpret_cast<TS*>(in.f3));
+    if (as_global(in.ctype)[c.o] > 0) {
       // frozen cell: f3 = f2 = f, and the new state is f
 #pragma unroll
       for (int k = 0; k < kPlanes; k++) f3b[in.plane * k + c.o] = src[in.plane * k + c.o];
       copy_frozen(in, src, out, cur, nxt, c.o);
       continue;
     }
-    s3_cell<TS, TC, TE>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), f3b, out, in.mom[nxt], c.o, step,
-                    MakeSeq<kQ>::type{});
+    s3_cell<TS, TC, TE>(in, src, as_global(in.mom[cur]), as_global(reinterpret_cast<const TS*>(in.dflux)), f3b, out,
+                        as_global(in.mom[nxt]), c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
@@ -587,8 +608,8 @@ __device__ __forceinline__ void s1(const Rec1<TC>& r, TC fv, TC hv, TC fe, TC he
 }
 
 template <typename TS, typename TC, typename TE, int... Is>
-__device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, const double* mom, TS* dst, long long o,
-                                        int step, Seq<Is...>) {
+__device__ __forceinline__ void s1_cell(const InstDev& in, gptr<const TS> src, gptr<const double> mom, gptr<TS> dst,
+                                        long long o, int step, Seq<Is...>) {
   const long long P = in.plane;
   CellState<TS> cs;
   load_state(in, src, mom, o, cs);
@@ -610,8 +631,8 @@ __device__ __forceinline__ void s1_cell(const InstDev& in, const TS* src, const 
 }
 
 template <typename TS, typename TC, typename TE>
-__global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                            const int* stepp) {
+__global__ __launch_bounds__(kNT) void k_s1(const InstDev* __restrict__ insts, const TileRef* __restrict__ tiles,
+                                            int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
@@ -621,22 +642,23 @@ __global__ __launch_bounds__(kNT) void k_s1(const InstDev* insts, const TileRef*
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par;
-    const TS* src = state_buf<TS>(in, cur);
-    TS* dst = reinterpret_cast<TS*>(in.f2);
-    if (in.ctype[c.o] > 0) {
+    const gptr<const TS> src = as_global(state_buf<TS>(in, cur));
+    const gptr<TS> dst = as_global(reinterpret_cast<TS*>(in.f2));
+    if (as_global(in.ctype)[c.o] > 0) {
 #pragma unroll
       for (int k = 0; k < kPlanes; k++) dst[in.plane * k + c.o] = src[in.plane * k + c.o];
       continue;
     }
-    s1_cell<TS, TC, TE>(in, src, in.mom[cur], dst, c.o, step, MakeSeq<kQ>::type{});
+    s1_cell<TS, TC, TE>(in, src, as_global(in.mom[cur]), dst, c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
 // Final update of the one-flux-stage steppers, METHOD 0 (RK1, cpuLBM.cpp:
 // 1696, 1707), 2 (EULER, :2155, :2165), 3 (EULER_EQ, :2245, :2255).
 template <int METHOD, typename TS, typename TC, typename TE, int... Is>
-__device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const double* mom, const TS* dfb, TS* out,
-                                         double* mom_out, long long o, int step, Seq<Is...>) {
+__device__ __forceinline__ void fin_cell(const InstDev& in, gptr<const TS> src, gptr<const double> mom,
+                                         gptr<const TS> dfb, gptr<TS> out, gptr<double> mom_out, long long o, int step,
+                                         Seq<Is...>) {
   const long long P = in.plane;
   CellState<TS> cs;
   load_state(in, src, mom, o, cs);
@@ -693,8 +715,8 @@ __device__ __forceinline__ void fin_cell(const InstDev& in, const TS* src, const
 }
 
 template <typename TS, typename TC, typename TE, int METHOD>
-__global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef* tiles, int ntiles, int par,
-                                             const int* stepp) {
+__global__ __launch_bounds__(kNT) void k_fin(const InstDev* __restrict__ insts, const TileRef* __restrict__ tiles,
+                                             int ntiles, int par, const int* stepp) {
   const int step = *stepp;
   for (int t = first_tile(); t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
@@ -704,14 +726,14 @@ __global__ __launch_bounds__(kNT) void k_fin(const InstDev* insts, const TileRef
     const PointCell c = point_cell(in, tr);
     if (!c.active) continue;
     const int cur = in.parity0 ^ par, nxt = 1 ^ cur;
-    const TS* src = state_buf<TS>(in, cur);
-    TS* out = reinterpret_cast<TS*>(in.state[nxt]);
-    if (in.ctype[c.o] > 0) {
+    const gptr<const TS> src = as_global(state_buf<TS>(in, cur));
+    const gptr<TS> out = as_global(reinterpret_cast<TS*>(in.state[nxt]));
+    if (as_global(in.ctype)[c.o] > 0) {
       copy_frozen(in, src, out, cur, nxt, c.o);
       continue;
     }
-    fin_cell<METHOD, TS, TC, TE>(in, src, in.mom[cur], reinterpret_cast<const TS*>(in.dflux), out, in.mom[nxt], c.o,
-                             step, MakeSeq<kQ>::type{});
+    fin_cell<METHOD, TS, TC, TE>(in, src, as_global(in.mom[cur]), as_global(reinterpret_cast<const TS*>(in.dflux)),
+                                 out, as_global(in.mom[nxt]), c.o, step, MakeSeq<kQ>::type{});
   }
 }
 
@@ -914,8 +936,8 @@ __device__ __forceinline__ void fetch_slots(SlotVals<Sh, TS>& V, const SlotSet<S
     const unsigned m = (unsigned)L.meta[j];
     const int d = (m & kMetaMirEW) ? ((m & kMetaMirNS) ? dboth : dew) : ((m & kMetaMirNS) ? dns : I);
     const long long o = L.off[j];
-    V.f[j] = plane<TS>(const_cast<void*>(src), P, d)[o];
-    V.h[j] = plane<TS>(const_cast<void*>(src), P, kQ + d)[o];
+    V.f[j] = as_global(plane<TS>(const_cast<void*>(src), P, d))[o];
+    V.h[j] = as_global(plane<TS>(const_cast<void*>(src), P, kQ + d))[o];
   }
 }
 
@@ -1224,8 +1246,8 @@ struct TileWork {
   bool solids;
   int method;
   const void* src;
-  TS* dst;
-  const TS* dfl;  // MODE 1: the stored stage-2 net fluxes
+  gptr<TS> dst;
+  gptr<const TS> dfl;  // MODE 1: the stored stage-2 net fluxes
   const double* alpha;
   long long P;
   double c;
@@ -1302,7 +1324,7 @@ __device__ __forceinline__ void flux_dir(FluxStage<Sh, TC, MOM>& S, const Region
   // G), which it needs sooner.
   TS d2f[Sh::Cells], d2h[Sh::Cells];
   if constexpr (MODE == 1) {
-    const TS* b = kStage3G<TS> ? W.dst : W.dfl;
+    const gptr<const TS> b = kStage3G<TS> ? (gptr<const TS>)W.dst : W.dfl;
     const long long pf = kStage3G<TS> ? P * I : P * (I - 1), ph = kStage3G<TS> ? P * (kQ + I) : P * (12 + I - 1);
 #pragma unroll
     for (int q = 0; q < Sh::Cells; q++) {
@@ -1431,8 +1453,8 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
   W.c = in.c;
   W.P = in.plane;
   W.src = MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]);
-  W.dst = MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux);
-  W.dfl = reinterpret_cast<const TS*>(in.dflux);
+  W.dst = as_global(MODE == 1 ? reinterpret_cast<TS*>(in.state[1 ^ in.parity0 ^ par]) : reinterpret_cast<TS*>(in.dflux));
+  W.dfl = as_global(reinterpret_cast<const TS*>(in.dflux));
   // own cells of this thread: cell ci = t + q NT at (ci % TX, ci / TX)
 #pragma unroll
   for (int q = 0; q < Sh::Cells; q++) {
@@ -1463,7 +1485,7 @@ __device__ __forceinline__ void flux_tile(FluxStage<Sh, TC, MOM>& S, StageBuf<Sh
     // the new state's conserved moments: those written by stage 3 (of
     // f - dF(f2)/2) minus half those of dF(f3)
     static_assert(MODE == 1, "moment sums belong to the stage-3 flux of FP32 storage");
-    double* mo = in.mom[1 ^ in.parity0 ^ par];
+    const gptr<double> mo = as_global(in.mom[1 ^ in.parity0 ^ par]);
 #pragma unroll
     for (int q = 0; q < Sh::Cells; q++) {
       if (!W.fluid[q]) continue;

@@ -16,7 +16,7 @@
 // every face and net flux is that of the tile kernels (face_flux, flux_dir).
 // The next row's loads are issued before the current row's faces, so their
 // latency hides behind them.
-// (Included inside namespace uqlbm.)
+// (Included inside namespace uqlbm, after the gptr global-pointer alias.)
 #pragma once
 
 // DPP controls of whole-wave shifts by one lane (GFX9 wave_shr:1 and
@@ -120,12 +120,6 @@ __device__ __forceinline__ void strip_wall(TC a, TC v, TC wf, TC wh, TC rf, TC r
     vh = a * v * wh;
   }
 }
-
-// Global-memory pointers (loads and stores the compiler issues as global,
-// not flat, operations, so that they count on the vector-memory counter
-// only and never hold up the scalar loads).
-template <typename T>
-using gptr = __attribute__((address_space(1))) T*;
 
 // The instance fields the strip kernels read, copied once per work item:
 // read through the instance table after the item's stores they would be
