@@ -1587,7 +1587,7 @@ __global__ __launch_bounds__(256) void k_phys(const InstDev* insts, const TileRe
 // is dropped where it is unused.)
 template <typename TS, typename TC, int MODE, bool MOM, typename Sh>
 __global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(8))) void k_flux(
-    const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
+    const InstDev* __restrict__ insts, const TileRef* __restrict__ tiles, int ntiles, int par, const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, TC, MOM> S;
   __shared__ StageBuf<Sh, TS> VA;
@@ -1597,7 +1597,7 @@ __global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(8))) voi
 
 template <int MODE, bool MOM, typename Sh>
 __global__ __launch_bounds__(Sh::NT) __attribute__((amdgpu_waves_per_eu(6))) void k_flux64(
-    const InstDev* insts, const TileRef* tiles, int ntiles, int par, const int* stepp) {
+    const InstDev* __restrict__ insts, const TileRef* __restrict__ tiles, int ntiles, int par, const int* stepp) {
   __shared__ Region<Sh> R;
   __shared__ FluxStage<Sh, double, MOM> S;
   __shared__ StageBuf<Sh, double> VA;

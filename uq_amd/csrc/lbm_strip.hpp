@@ -480,8 +480,8 @@ __device__ __forceinline__ void strip_item(const InstDev& in, const TileRef& sr,
 // 73 slots) and run slower (4096^2 Riemann stage-2 and stage-3 flux 2.19
 // and 2.82 ms at 4 waves, 1.77 and 2.18 ms at 3).
 template <typename TS, typename TC, int MODE, bool SOLID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_strip(const InstDev* insts, const TileRef* strips, int nstrips, int par,
-                                               const int* stepp) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_strip(
+    const InstDev* __restrict__ insts, const TileRef* __restrict__ strips, int nstrips, int par, const int* stepp) {
   const int step = *stepp;
   const int wave = __builtin_amdgcn_readfirstlane(first_tile() * 4 + (int)(threadIdx.x >> 6));
   const int nw = gridDim.x * 4;
