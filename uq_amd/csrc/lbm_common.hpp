@@ -110,8 +110,10 @@ constexpr bool kStage3G = sizeof(TS) == 4;
 
 // One tile of one instance: the unit of work of a block.
 // Strip work items of the strip flux kernels (TileRef with tx = strip index,
-// ty = first row, flags = kTileSolid | row count << 8): kStripW own columns
-// (with a 3-column halo on each side, 64 lanes) by up to kStripH rows.
+// ty = first row, flags = kTileSolid | kStripDesc | row count << 8):
+// kStripW own columns (with a 3-column halo on each side, 64 lanes) by up to
+// kStripH rows; kStripDesc: the item takes its rows from the last to the
+// first.
 constexpr int kStripW = 58, kStripH = 32;
 
 struct TileRef {
@@ -121,7 +123,7 @@ struct TileRef {
                        // kTilePhysical: an own cell needs the reference's physical addressing
 };
 
-enum : int { kTileSolid = 1, kTilePhysical = 2 };
+enum : int { kTileSolid = 1, kTilePhysical = 2, kStripDesc = 4 };
 
 // Wall cell that needs a closure factor (has a non-solid axis neighbour).
 struct WallRef {

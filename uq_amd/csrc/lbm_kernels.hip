@@ -1935,12 +1935,13 @@ void flux_tile_dims(int shape, int* w, int* h) {
 int tile_halo() { return kH; }
 int dflux_planes() { return kNDF; }
 
-static int grid_for(const void* kernel, int threads) {
+static int grid_for(const void* kernel, int threads, int max_per = 0) {
   int dev = 0, cus = 256, per = 4;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+  if (max_per > 0 && per > max_per) per = max_per;
   long long g = (long long)cus * per;
   g -= g % 8;
   return (int)(g < 8 ? 8 : g);
@@ -2019,8 +2020,12 @@ struct StripGrids {
   static const StripGrids& get() {
     static const StripGrids g = [] {
       StripGrids r;
-      r.fast = grid_for((const void*)k_strip<TS, TC, MODE, false>, 256);
-      r.solid = grid_for((const void*)k_strip<TS, TC, MODE, true>, 256);
+      // FP64 arithmetic: three 4-wave blocks per CU (three waves per SIMD)
+      // even where the registers would allow four, which run slower (4096^2
+      // Riemann stage-2 flux 2.09 ms at four waves, 1.67 ms at three)
+      const int per = sizeof(TC) == 8 ? 3 : 0;
+      r.fast = grid_for((const void*)k_strip<TS, TC, MODE, false>, 256, per);
+      r.solid = grid_for((const void*)k_strip<TS, TC, MODE, true>, 256, per);
       return r;
     }();
     return g;

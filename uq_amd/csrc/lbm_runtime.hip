@@ -578,7 +578,8 @@ static std::vector<TileRef> build_flux_tiles(const Solver& s, int shape) {
 // it: periodic, mirrored or clamped) holds a solid cell; consecutive rows
 // of the same kind form an item, so that only the rows near a solid take
 // the solid path. TileRef: tx = strip index, ty = first row,
-// flags = kTileSolid (solid rows) | row count << 8.
+// flags = kTileSolid (solid rows) | kStripDesc (every second item of a
+// column) | row count << 8.
 static void build_strips(const Solver& s, std::vector<TileRef>& fast, std::vector<TileRef>& solid) {
   fast.clear();
   solid.clear();
@@ -607,11 +608,14 @@ static void build_strips(const Solver& s, std::vector<TileRef>& fast, std::vecto
         rowsolid[y] = any;
       }
     }
-    for (int y = 0; y < s.ny;) {
+    // the items of a strip column alternate between ascending and
+    // descending row order (kStripDesc), so that neighbours one above the
+    // other read the rows they share at about the same time
+    for (int y = 0, k = 0; y < s.ny; k++) {
       const bool sol = rowsolid[y];
       int n = 1;
       while (y + n < s.ny && n < kStripH && (bool)rowsolid[y + n] == sol) n++;
-      (sol ? solid : fast).push_back(TileRef{0, sx, y, (sol ? kTileSolid : 0) | (n << 8)});
+      (sol ? solid : fast).push_back(TileRef{0, sx, y, (sol ? kTileSolid : 0) | ((k & 1) ? kStripDesc : 0) | (n << 8)});
       y += n;
     }
   }

@@ -140,12 +140,21 @@ struct StripInst {
 };
 
 // One direction over the rows of a work item. SX, SY: the signs of the
-// direction's lattice velocity; SOLID: the item's region holds a solid cell;
-// W5: WENO5 faces (else NND).
+// direction's lattice velocity in the march's row order; SOLID: the item's
+// region holds a solid cell; W5: WENO5 faces (else NND). flip: the march
+// takes the item's rows from the last to the first, as rows y0 .. y0 +
+// nrows - 1 of the item turned upside down (row y of the march is row
+// 2 y0 + nrows - 1 - y of the instance), called with SY the negated sign
+// of the direction: each face then has the same upwind owner and the same
+// stencil values in the same order as in the ascending march, so the two
+// orders give the same values bit for bit.
 template <typename TS, typename TC, int MODE, bool SOLID, bool W5, int SX, int SY>
 __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLane<TC>& C, TC rf, TC rh, int I,
-                                          int y0, int nrows) {
+                                          int y0, int nrows, int flip) {
   constexpr bool HX = SX != 0, HY = SY != 0;
+  // the instance row of row y of the march
+  const int yflip = 2 * y0 + nrows - 1;
+  auto prow = [&](int y) { return flip ? yflip - y : y; };
   const int pitch = in.pitch, ny = in.ny;
   const int aex = c_lat_x[I] > 0 ? c_lat_x[I] : -c_lat_x[I];
   const int aey = c_lat_y[I] > 0 ? c_lat_y[I] : -c_lat_y[I];
@@ -178,7 +187,7 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   };
   auto load = [&](int yl) {
     unsigned my;
-    const int ro = map_row(yl, my);
+    const int ro = map_row(prow(yl), my);
     const bool m = nsmir && my;
     SPos<TC> p;
     p.f = TC((m ? pf1 : pf0)[ro]);
@@ -190,7 +199,7 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   auto clamped_row = [&](int yl) {
     if (in.inner) return 0;
     unsigned my, cy;
-    (void)map_axis(yl, ny, in.py, in.ms, in.mn, my, cy);
+    (void)map_axis(prow(yl), ny, in.py, in.ms, in.mn, my, cy);
     return (int)cy;
   };
 
@@ -277,7 +286,7 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   auto fetch = [&](int y) {
     Pre r;
     r.p = load(HY ? y + 3 : y);
-    r.ro = y * pitch;
+    r.ro = prow(y) * pitch;
     r.type = 1;
     r.f = r.h = TS(0);
     if (C.own) {
@@ -287,7 +296,7 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
         r.h = dh2[r.ro];
       }
     }
-    r.rdy = TC(in.rdyp[y * in.rdy_stride]);
+    r.rdy = TC(in.rdyp[prow(y) * in.rdy_stride]);
     return r;
   };
   // a row's results, stored at the start of the next half iteration
@@ -410,29 +419,37 @@ __device__ __forceinline__ void strip_dir(const StripInst<TS>& in, const StripLa
   store(R);
 }
 
-// The twelve directions of a work item, one flux method.
+// The twelve directions of a work item, one flux method. desc: the item
+// takes its rows from the last to the first in the directions with a y
+// component. Items alternate between the two orders down each strip column
+// (kStripDesc), so that two items one above the other reach the rows they
+// share (each one's rows next to the other, read by the other as its y
+// halo) at about the same time: the first read of such a row leaves it in
+// L2 for the second.
 template <typename TS, typename TC, int MODE, bool SOLID, bool W5>
 __device__ __forceinline__ void strip_dirs(const InstDev& in, const StripInst<TS>& si, const StripLane<TC>& C, int y0,
-                                           int nrows) {
+                                           int nrows, bool desc) {
   for (int I = 1; I < kQ; I++) {
     const int sx = c_lat_x[I] > 0 ? 1 : (c_lat_x[I] < 0 ? -1 : 0);
-    const int sy = c_lat_y[I] > 0 ? 1 : (c_lat_y[I] < 0 ? -1 : 0);
+    const int sy0 = c_lat_y[I] > 0 ? 1 : (c_lat_y[I] < 0 ? -1 : 0);
+    const int fl = desc && sy0 != 0;
+    const int sy = fl ? -sy0 : sy0;  // the sign in the march's row order
     // reference equilibrium of direction I for the wall faces of FP32
     // storage (a mirror image of I has the same reference: it depends on |e|)
     const TC rf = sizeof(TS) == 4 ? TC(in.ref[I]) : TC(0);
     const TC rh = sizeof(TS) == 4 ? TC(in.ref[kQ + I]) : TC(0);
     if (sy == 0) {
-      if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, 0>(si, C, rf, rh, I, y0, nrows);
-      else strip_dir<TS, TC, MODE, SOLID, W5, -1, 0>(si, C, rf, rh, I, y0, nrows);
+      if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, 0>(si, C, rf, rh, I, y0, nrows, 0);
+      else strip_dir<TS, TC, MODE, SOLID, W5, -1, 0>(si, C, rf, rh, I, y0, nrows, 0);
     } else if (sx == 0) {
-      if (sy > 0) strip_dir<TS, TC, MODE, SOLID, W5, 0, 1>(si, C, rf, rh, I, y0, nrows);
-      else strip_dir<TS, TC, MODE, SOLID, W5, 0, -1>(si, C, rf, rh, I, y0, nrows);
+      if (sy > 0) strip_dir<TS, TC, MODE, SOLID, W5, 0, 1>(si, C, rf, rh, I, y0, nrows, fl);
+      else strip_dir<TS, TC, MODE, SOLID, W5, 0, -1>(si, C, rf, rh, I, y0, nrows, fl);
     } else if (sy > 0) {
-      if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, 1>(si, C, rf, rh, I, y0, nrows);
-      else strip_dir<TS, TC, MODE, SOLID, W5, -1, 1>(si, C, rf, rh, I, y0, nrows);
+      if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, 1>(si, C, rf, rh, I, y0, nrows, fl);
+      else strip_dir<TS, TC, MODE, SOLID, W5, -1, 1>(si, C, rf, rh, I, y0, nrows, fl);
     } else {
-      if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, -1>(si, C, rf, rh, I, y0, nrows);
-      else strip_dir<TS, TC, MODE, SOLID, W5, -1, -1>(si, C, rf, rh, I, y0, nrows);
+      if (sx > 0) strip_dir<TS, TC, MODE, SOLID, W5, 1, -1>(si, C, rf, rh, I, y0, nrows, fl);
+      else strip_dir<TS, TC, MODE, SOLID, W5, -1, -1>(si, C, rf, rh, I, y0, nrows, fl);
     }
   }
 }
@@ -442,6 +459,7 @@ template <typename TS, typename TC, int MODE, bool SOLID>
 __device__ __forceinline__ void strip_item(const InstDev& in, const TileRef& sr, int par) {
   const int lane = threadIdx.x & 63;
   const int x0 = sr.tx * kStripW, y0 = sr.ty, nrows = sr.flags >> 8;
+  const bool desc = (sr.flags & kStripDesc) != 0;
   StripInst<TS> si;
   si.src = (gptr<const TS>)(MODE == 0 ? in.f2 : (MODE == 1 ? in.f3 : in.state[in.parity0 ^ par]));
   si.dst = (gptr<TS>)(MODE == 1 ? in.state[1 ^ in.parity0 ^ par] : in.dflux);
@@ -470,8 +488,8 @@ __device__ __forceinline__ void strip_item(const InstDev& in, const TileRef& sr,
   C.clampR = (int)cr;
   C.own = lane >= kH && lane < kH + kStripW && C.xl < in.nx;
   C.rdx = C.own ? TC(in.rdxs ? in.rdxs[C.xl] : in.rdx) : TC(0);
-  if (in.flux_method == 1) strip_dirs<TS, TC, MODE, SOLID, true>(in, si, C, y0, nrows);
-  else strip_dirs<TS, TC, MODE, SOLID, false>(in, si, C, y0, nrows);
+  if (in.flux_method == 1) strip_dirs<TS, TC, MODE, SOLID, true>(in, si, C, y0, nrows, desc);
+  else strip_dirs<TS, TC, MODE, SOLID, false>(in, si, C, y0, nrows, desc);
 }
 
 // Persistent grid of 4-wave blocks; wave w of the (XCD-renumbered) grid
