@@ -7,7 +7,9 @@ doubly periodic (shear wave), periodic with solid walls (Couette, WENO5 and
 NND), clamped edges (Riemann), a north/south mirror with a clamp opposite
 and a solid block (obstacle), east/west mirrors, a stretched grid
 (per-column dt/dx, per-row dt/dy), the one-stage steppers (RK1, Euler), grids
-whose sides are no multiple of the strip width or height, instances
+whose sides are no multiple of the strip width or height (taller than one
+row block: the items of a strip column alternate between ascending and
+descending row order, so both orders are compared), instances
 narrower than one strip and shorter than one row block, and mixed
 precision (FP32 storage: the stage-2 flux runs on strips, the
 moment-summing stage-3 flux stays on tiles).
