@@ -63,14 +63,15 @@ def kernel_bytes_per_lup(es: int) -> dict:
     moments in a first pass whose lines the second pass finds in L2) and
     writes f3, its rest direction straight into the new state (26 + 24 +
     26); flux3 reads f3 and the stage-2 net fluxes and writes the new state
-    f3 + (dF2 - dF3)/2 (24 + 24 + 24). FP32 storage: s3 writes f3 and
-    G = f3 + dF2/2 (26 + 24 + 52) and flux3 reads f3 and G (24 + 24 + 24),
+    f3 + (dF2 - dF3)/2 (24 + 24 + 24). FP32 storage: s3 writes f3 but for
+    its rest direction and G = f3 + dF2/2 (26 + 24 + 24 + 26) and flux3
+    reads f3 and G (24 + 24 + 24),
     and the 4 FP64 conserved-moment planes of the state add 32 B to s12
     (read), 64 B to s3 (read, written for G) and 64 B to flux3 (read and
     written). Halo re-reads, the cell-type map and the wall-closure kernels
     are excluded."""
     if es == 4:
-        return {"s12": 50 * es + 32, "flux2": 48 * es, "s3": 102 * es + 64, "flux3": 72 * es + 64}
+        return {"s12": 50 * es + 32, "flux2": 48 * es, "s3": 100 * es + 64, "flux3": 72 * es + 64}
     return {"s12": 50 * es, "flux2": 48 * es, "s3": 76 * es, "flux3": 72 * es}
 
 

@@ -503,8 +503,10 @@ __device__ __forceinline__ void s3_cell(const InstDev& in, gptr<const TS> src, g
     if constexpr (kStage3G<TS>) {
       const TC gf = (Pf + TC(0.5) * dF) + af3 * (fe3 - f3);
       const TC gh = ((Qh + TC(0.5) * dH) + ah3 * (he3 - h3)) - hdt * A3;
-      __builtin_nontemporal_store(TS(f3), &f3b[P * I + o]);
-      __builtin_nontemporal_store(TS(h3), &f3b[P * (kQ + I) + o]);
+      if constexpr (I != 0) {  // the rest direction of f3 has no flux and no wall term: nothing reads it
+        __builtin_nontemporal_store(TS(f3), &f3b[P * I + o]);
+        __builtin_nontemporal_store(TS(h3), &f3b[P * (kQ + I) + o]);
+      }
       __builtin_nontemporal_store(TS(gf), &out[P * I + o]);
       __builtin_nontemporal_store(TS(gh), &out[P * (kQ + I) + o]);
     } else {
